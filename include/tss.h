@@ -1,0 +1,255 @@
+/*
+ * tss.h -- C ABI of the MI355X two-scale WCSPH solver (libtss.so).
+ *
+ * This is the drop-in boundary for the reference's solver path
+ * (segfault11/TwoScaleSplit, TwoScale2D/WCSPHSolver.{h,cu} and
+ * TwoScale2D/ParticleSystem.{h,cpp}; "R/" below = TwoScale2D/). The C++ class
+ * API of the reference (WCSPHConfig, WCSPHSolver, ParticleSystem) is kept by
+ * the header-only facade twoscalesplit_amd/host/WCSPHSolver.h on top of these
+ * entry points.
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a negative TSS_E_* code on failure;
+ *    tss_last_error() then returns a thread-local message (the reference
+ *    throws std::runtime_error from UTIL::ThrowException, R/stdutil.h:22-33,
+ *    and CUDA_SAFE_CALL, R/CUDA/cuda.h:37-51; the facade rethrows).
+ *  - Handles are independent: any number of solvers and batches may coexist
+ *    (the reference allows one per process, R/WCSPHSolver.cu:23-48,2728). A
+ *    handle is not thread-safe; the caller serialises calls on it.
+ *  - The library owns all device memory behind a handle. tss_read/tss_write
+ *    copy between host memory and the device state and synchronise the
+ *    handle's stream; tss_step does not synchronise.
+ *  - Positions, velocities and accelerations are interleaved (x, y) float
+ *    pairs indexed by particle id, like the reference's float2 arrays
+ *    (R/ParticleSystem.h:56-77). High-res child k of low particle p has id
+ *    4p+k (R/WCSPHSolver.cu:971-995).
+ */
+#ifndef TSS_H
+#define TSS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSS_OK 0
+#define TSS_E_INVALID -1   /* bad argument / configuration          */
+#define TSS_E_HIP -2       /* HIP runtime error                     */
+#define TSS_E_NOMEM -3     /* device allocation or budget exhausted */
+#define TSS_E_STATE -4     /* call not valid in the handle's state  */
+
+/* Two-scale adaptivity modes. */
+#define TSS_MODE_PLAIN 0     /* single level: no high-res particles          */
+#define TSS_MODE_REFERENCE 1 /* the reference's live rule: split an active low
+                                particle iff its predicted x > split_x;
+                                children stay transient (R:1834-1863)        */
+#define TSS_MODE_BAND 2      /* split inside the free-surface band, promote
+                                relaxed groups, merge groups outside it       */
+
+#define TSS_DT_FIXED 0
+#define TSS_DT_CFL 1
+
+/* Solver configuration. The first block is the reference's WCSPHConfig
+ * (R/WCSPHSolver.h:14-35); the rest are the constants the reference
+ * hard-codes (gravity R:1805, the split rule R:1835) and the two-scale and
+ * time-step controls of the method spec. */
+typedef struct tss_config {
+    float domain_origin[2];
+    float domain_end[2];
+    int domain_dims[2];       /* set by tss_config_init: ceil(L/h) in FP32 */
+    int domain_dims_high[2];  /* ceil(L/H)                                 */
+    float effective_radius;   /* h: support radius of the low level        */
+    float effective_radius_high; /* H                                       */
+    float rest_density;
+    float tait_coefficient;   /* B                                         */
+    float speed_sound;
+    float alpha;              /* artificial viscosity                      */
+    float tension_coefficient;
+    float time_step;          /* initial (or fixed) dt                     */
+    float gravity;            /* 9.81                                      */
+    float mass_low, mass_high, mass_boundary;
+    int mode;                 /* TSS_MODE_*                                */
+    float split_x;            /* reference rule threshold (0.5)            */
+    float surface_ratio;      /* low surface iff rho < surface_ratio * rho0 */
+    float surface_ratio_high; /* high: rho < surface_ratio_high * rho0;
+                                 default surface_ratio * kappa(H/h), kappa =
+                                 bulk value of the high-level density, whose
+                                 blended kernel is cut at r <= H (R:439-474) */
+    float band_radius;        /* split band distance (<= h)                */
+    float merge_radius;       /* merge when all children are farther       */
+    int promote_min_age, promote_max_age;
+    float promote_fluc;
+    int cap_high;             /* max live high particles (<= 4*cap_low)     */
+    int dt_mode;              /* TSS_DT_*                                  */
+    float cfl, dt_max;
+} tss_config;
+
+typedef struct tss_solver tss_solver;
+typedef struct tss_batch tss_batch;
+
+/* Fields for tss_read / tss_write / tss_device_ptr (per scene). */
+enum tss_field {
+    TSS_POS_L = 0,   /* float[2*cap_low]   by low id                     */
+    TSS_VEL_L,       /* float[2*cap_low]                                 */
+    TSS_ACC_L,       /* float[2*cap_low]   acceleration of the last step */
+    TSS_RHO_L,       /* float[cap_low]     density of the last step      */
+    TSS_P_L,         /* float[cap_low]     pressure of the last step     */
+    TSS_STATE_L,     /* uint8[cap_low]     0 active 1 transient 2 deleted */
+    TSS_AGE_L,       /* int32[cap_low]     steps since split             */
+    TSS_VIS_L,       /* float[cap_low]     min(|rho-rho0|,400)/400       */
+    TSS_POS_H,       /* float[8*cap_low]   by high id                    */
+    TSS_VEL_H,
+    TSS_ACC_H,
+    TSS_RHO_H,       /* float[4*cap_low]                                 */
+    TSS_P_H,
+    TSS_STATE_H,     /* uint8[4*cap_low]   1 transient child 0 promoted  */
+    TSS_VIS_H,
+    TSS_LIST_L,      /* int32[n_low]   active low ids (next step's list)  */
+    TSS_LIST_H,      /* int32[n_high]                                    */
+    TSS_LIST_G,      /* int32[n_groups] parents that own children        */
+    TSS_SORTED_L,    /* int32[n_low_last]  last step's hash-sorted ids    */
+    TSS_SORTED_H,    /* int32[n_high_last]                               */
+    TSS_KEYS_L,      /* int32[n_low_last]  sorted cell hashes            */
+    TSS_KEYS_H,
+    TSS_CELL_START_L,/* int32[cells_low]   -1 for empty cells            */
+    TSS_CELL_END_L,
+    TSS_CELL_START_H,/* int32[cells_high]                                */
+    TSS_CELL_END_H,
+    TSS_BND_SORTED,  /* int32[n_bnd] boundary ids in hash order          */
+    TSS_BND_START,   /* int32[cells_low]                                 */
+    TSS_BND_END,
+    TSS_POS_B,       /* float[2*n_bnd] boundary positions by id          */
+    TSS_DT,          /* float[1] dt of the next step                     */
+    TSS_FIELD_COUNT
+};
+
+/* Per-scene counters (tss_counts / tss_batch_counts). */
+typedef struct tss_counts_t {
+    int n_low, n_high, n_groups;          /* lists of the next step          */
+    int n_low_last, n_high_last;          /* lists the last step advanced    */
+    int n_split, n_promote, n_merge;      /* events of the last step         */
+    int n_dissolve;                       /* transient groups removed        */
+    int step;                             /* completed steps                 */
+    int cap_low, n_bnd;
+    int cells_low, cells_high;
+} tss_counts_t;
+
+/* ---- configuration ---------------------------------------------------- */
+/* Replaces WCSPHConfig::WCSPHConfig (R/WCSPHSolver.cu:2287-2332): fills the
+ * WCSPHConfig fields, derives the grid dimensions (ceil of FP32 quotient)
+ * and sets the remaining fields to defaults (gravity 9.81, split_x 0.5,
+ * surface_ratio 0.9, surface_ratio_high 0.9 kappa(H/h), band/merge radius
+ * h, plain mode, fixed dt, cap 0).
+ * TSS_E_INVALID if xs >= xe or ys >= ye, like the reference's exception. */
+int tss_config_init(tss_config* cfg, float xs, float ys, float xe, float ye,
+                    float effective_radius, float effective_radius_high,
+                    float rest_density, float tait_coefficient,
+                    float speed_sound, float alpha,
+                    float tension_coefficient, float time_step);
+
+/* ---- single-scene solver ------------------------------------------------ */
+/* Replaces WCSPHSolver::WCSPHSolver + Bind (R/WCSPHSolver.cu:2391-2520,
+ * 2537-2639) and the particle uploads of CreateParticleBox /
+ * CreateParticleBoxCanvas (R/ParticleSystem.cpp:123-179): n_low fluid
+ * particles (low ids 0..n_low-1, capacity cap_low >= n_low) and n_bnd
+ * static boundary particles; builds the boundary grid once. */
+int tss_create(const tss_config* cfg, const float* low_xy, int n_low,
+               int cap_low, const float* bnd_xy, int n_bnd,
+               tss_solver** out);
+/* Replaces WCSPHSolver::AdvanceTS (R/WCSPHSolver.cu:2726-2810): n_steps
+ * steps, enqueued on the handle's stream, no host synchronisation. */
+int tss_step(tss_solver* s, int n_steps);
+/* Same n_steps through a captured hipGraph of one step (replayed). */
+int tss_step_graph(tss_solver* s, int n_steps);
+/* Counters (synchronising). Replaces ParticleSystem::GetNumParticles /
+ * GetMaxParticles (R/ParticleSystem.h:35-37). */
+int tss_counts(tss_solver* s, tss_counts_t* out);
+/* Synchronising read-back / upload of a field (see enum tss_field).
+ * Replaces the Positions/Velocities/Densities/Pressures/VisualQuantities/
+ * ParticleIDs accessors (R/ParticleSystem.h:27-32) + Map/Unmap
+ * (R/ParticleSystem.cpp:78-98). bytes must not exceed the field's size. */
+int tss_read(tss_solver* s, int field, void* dst, size_t bytes);
+int tss_write(tss_solver* s, int field, const void* src, size_t bytes);
+/* Sets the list counts (n_low, n_high, n_groups) after tss_write of lists:
+ * re-seeding a state for tests and restart. */
+int tss_set_counts(tss_solver* s, int n_low, int n_high, int n_groups);
+/* Device pointer of a field (valid until tss_destroy), replacing GL
+ * interop for device-side consumers. */
+int tss_device_ptr(tss_solver* s, int field, void** out);
+int tss_synchronize(tss_solver* s);
+/* FP64 conservation sums of the last step (mass, px, py, kinetic,
+ * potential) and a rigorous bound of each sum's rounding error. */
+int tss_conservation(tss_solver* s, double out[5], double err_bound[5]);
+/* Neighbour sets of the current state: per listed particle of a level (in
+ * list order), 5 categories of counts and order-independent id checksums
+ * (see oracle's orc_neighbors for the categories). Builds the grid of the
+ * current lists; does not advance. counts/sums hold 5*n entries. */
+int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums,
+                  size_t n);
+/* Per-phase device time of the last tss_step call with timing enabled (ms):
+ * [0] grid build (hash+sort+ranges+reorder) [1] density [2] force
+ * [3] adapt+compaction [4] reductions. */
+int tss_enable_timing(tss_solver* s, int on);
+int tss_phase_times(tss_solver* s, float out_ms[5]);
+/* Replaces WCSPHSolver::~WCSPHSolver (R/WCSPHSolver.cu:2522-2535). */
+void tss_destroy(tss_solver* s);
+
+/* ---- batched scenes ----------------------------------------------------- */
+/* Many independent scenes advanced by one launch per phase. Capacities are
+ * totals over resident scenes; tss_batch_memory_bytes gives the device
+ * bytes such a batch holds. No counterpart in the reference, which allows
+ * one solver per process (R/WCSPHSolver.cu:23-48,2728). */
+typedef struct tss_batch_desc {
+    int max_scenes;
+    long long cap_low;     /* total low-particle capacity (children 4x) */
+    long long cap_bnd;     /* total boundary particles                  */
+    long long cells_low;   /* total low-grid cells                      */
+    long long cells_high;  /* total high-grid cells                     */
+} tss_batch_desc;
+
+int tss_batch_memory_bytes(const tss_batch_desc* d, long long* bytes);
+int tss_batch_create(const tss_batch_desc* d, tss_batch** out);
+/* Admits a scene (same arguments as tss_create) between steps. Returns
+ * TSS_E_NOMEM when the batch's capacities cannot hold it. */
+int tss_batch_add_scene(tss_batch* b, const tss_config* cfg,
+                        const float* low_xy, int n_low, int cap_low,
+                        const float* bnd_xy, int n_bnd, int* scene_id);
+/* Retires a scene; its slots are reusable by the next admission. */
+int tss_batch_remove_scene(tss_batch* b, int scene_id);
+int tss_batch_step(tss_batch* b, int n_steps);
+int tss_batch_step_graph(tss_batch* b, int n_steps);
+int tss_batch_counts(tss_batch* b, int scene_id, tss_counts_t* out);
+int tss_batch_read(tss_batch* b, int scene_id, int field, void* dst,
+                   size_t bytes);
+int tss_batch_conservation(tss_batch* b, int scene_id, double out[5],
+                           double err_bound[5]);
+int tss_batch_synchronize(tss_batch* b);
+int tss_batch_num_scenes(tss_batch* b, int* n);
+/* Launch-shape tuning: block sizes of the neighbour kernels. */
+int tss_batch_set_tuning(tss_batch* b, int density_block, int force_block);
+void tss_batch_destroy(tss_batch* b);
+
+/* ---- scene generator (host) ----------------------------------------------- */
+/* Seeded dam-break scene: fluid block of nx*ny particles on a lattice of
+ * spacing dx at ((i+0.5)dx, (j+0.5)dx) in row order (j outer, like
+ * CreateParticleBox, R/ParticleSystem.cpp:123-148), each jittered by
+ * U(-jitter*dx, jitter*dx) from SplitMix64 keyed by (seed, scene, index);
+ * plus `layers` wall layers around the domain [0, nxd*dx] x [0, nyd*dx]
+ * (CreateParticleBoxCanvas, R/ParticleSystem.cpp:150-179). Arrays are
+ * caller-owned: low_xy[2*nx*ny], bnd_xy[2*tss_canvas_count(...)]. */
+int tss_canvas_count(int nxd, int nyd, int layers);
+int tss_scene_dam_break(uint64_t seed, uint64_t scene, int nx, int ny,
+                        float dx, float jitter, int nxd, int nyd, int layers,
+                        float* low_xy, float* bnd_xy);
+/* SplitMix64 output stream (for tests of the generator). */
+uint64_t tss_splitmix64_next(uint64_t* state);
+
+const char* tss_last_error(void);
+const char* tss_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,223 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the oracle.
+
+Each case prints its name, scene and step before touching the GPU.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from parity import check_step
+from twoscalesplit_amd import MODE_BAND, MODE_REFERENCE, Batch, Solver
+from twoscalesplit_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _solver(sc):
+    return Solver(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+
+
+def _run_checks(sc, steps_between, report):
+    sol = _solver(sc)
+    try:
+        for k, adv in enumerate(steps_between):
+            if adv:
+                _say(sc.name, "advance", adv)
+                sol.step(adv)
+            _say(sc.name, "check step", sol.counts()["step"])
+            check_step(sol, sc.cfg, sc.bnd_xy, report)
+    finally:
+        sol.close()
+
+
+def test_c1_plain_parity():
+    sc = scenes.c1_plain()
+    rep = []
+    _run_checks(sc, [0, 0, 20, 80], rep)
+    print(json.dumps(rep))
+
+
+def test_c2_two_scale_parity():
+    sc = scenes.c2_two_scale()
+    rep = []
+    _run_checks(sc, [0, 0, 0, 30], rep)
+    print(json.dumps(rep))
+    assert rep[0]["events"][0] > 0, "the first step must split the surface band"
+
+
+def test_reference_rule_parity():
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_REFERENCE,
+                                split_x=0.08)
+    rep = []
+    _run_checks(sc, [0, 0, 0, 10, 10], rep)
+    print(json.dumps(rep))
+    assert sum(r["events"][0] for r in rep) > 0
+
+
+def test_promote_merge_dissolve_parity():
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND)
+    sc.cfg.promote_min_age = 1
+    sc.cfg.promote_max_age = 3
+    sc.cfg.promote_fluc = 1.0
+    rep = []
+    _run_checks(sc, [0, 0, 0, 0, 0, 0, 10, 0, 0], rep)
+    print(json.dumps(rep))
+    ev = np.array([r["events"] for r in rep]).sum(axis=0)
+    assert ev[0] > 0 and ev[1] > 0, ev
+
+
+def test_cfl_dt_parity():
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND, dt_cfl=True)
+    rep = []
+    _run_checks(sc, [0, 0, 5], rep)
+    print(json.dumps(rep))
+
+
+def test_neighbour_sets_bit_exact(oracle):
+    sc = scenes.c2_two_scale()
+    _say(sc.name, "neighbour sets after 3 steps")
+    sol = _solver(sc)
+    try:
+        sol.step(3)
+        snap = sol.snapshot()
+        o = oracle.Scene(oracle.params_from_config(sc.cfg), sc.cap_low,
+                         sc.bnd_xy)
+        o.load_snapshot(snap)
+        g = o.grid()
+        for level in (0, 1):
+            c_gpu, s_gpu = sol.neighbors(level)
+            c_orc, s_orc = o.neighbors(g, level)
+            assert len(c_gpu) > 0
+            np.testing.assert_array_equal(c_gpu, c_orc)
+            np.testing.assert_array_equal(s_gpu, s_orc)
+    finally:
+        sol.close()
+
+
+def test_conservation_fp64_bound():
+    sc = scenes.c2_two_scale()
+    _say(sc.name, "conservation sums")
+    sol = _solver(sc)
+    try:
+        sol.step(5)
+        pre = sol.snapshot()
+        sol.step(1)
+        post = sol.snapshot()
+        sums, bound = sol.conservation()
+        sl = sol.read("SORTED_L")
+        sh = sol.read("SORTED_H")
+        c = sc.cfg
+        terms = [[] for _ in range(5)]
+        for ids, lvl in ((sl, 0), (sh, 1)):
+            m = float(c.mass_high if lvl else c.mass_low)
+            P = post["POS_H" if lvl else "POS_L"][ids].astype(np.float64)
+            V = post["VEL_H" if lvl else "VEL_L"][ids].astype(np.float64)
+            if lvl:
+                real = pre["STATE_H"][ids] == 0
+                P, V = P[real], V[real]
+            terms[0] += [m] * len(P)
+            terms[1] += list(m * V[:, 0])
+            terms[2] += list(m * V[:, 1])
+            terms[3] += list(0.5 * (m * (V[:, 0] ** 2 + V[:, 1] ** 2)))
+            terms[4] += list((m * float(c.gravity)) * P[:, 1])
+        for q in range(5):
+            exact = math.fsum(terms[q])
+            assert abs(sums[q] - exact) <= bound[q] + 1e-300, (q, sums[q],
+                                                                exact, bound[q])
+    finally:
+        sol.close()
+
+
+def test_graph_replay_equals_eager():
+    sc = scenes.c2_two_scale()
+    _say(sc.name, "graph replay vs eager, 10 steps")
+    a, b = _solver(sc), _solver(sc)
+    try:
+        a.step(10)
+        b.step_graph(10)
+        sa, sb = a.snapshot(), b.snapshot()
+        for k in sa:
+            if k == "counts":
+                assert sa[k] == sb[k]
+            else:
+                np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+        # replays interleaved with eager steps and reads stay identical
+        a.step(4)
+        b.step_graph(2)
+        b.step(2)
+        for k in ("POS_L", "POS_H", "RHO_L", "LIST_L", "LIST_H"):
+            np.testing.assert_array_equal(a.read(k), b.read(k), err_msg=k)
+    finally:
+        a.close()
+        b.close()
+
+
+def _batch_for(scs):
+    cap = sum(s.cap_low for s in scs)
+    nb = sum(len(s.bnd_xy) for s in scs)
+    cl = sum(s.cells[0] for s in scs)
+    ch = sum(s.cells[1] for s in scs)
+    return Batch(len(scs) + 1, cap + 100, nb + 10, cl + 10, ch + 10)
+
+
+def test_batch_alone_equals_mix():
+    scs = [scenes.small_two_scale(nx=16, ny=24, mode=MODE_BAND),
+           scenes.c2_two_scale(),
+           scenes.small_two_scale(nx=20, ny=12, mode=MODE_REFERENCE,
+                                  split_x=0.05)]
+    scs[0].cfg.promote_min_age = 1
+    scs[0].cfg.promote_max_age = 3
+    _say("batch of", [s.name for s in scs], "vs alone, 12 steps")
+    b = _batch_for(scs)
+    try:
+        ids = [b.add_scene(s.cfg, s.low_xy, s.cap_low, s.bnd_xy) for s in scs]
+        b.step(5)
+        b.step_graph(7)
+        for s, sid in zip(scs, ids):
+            alone = _solver(s)
+            alone.step(12)
+            for k in ("POS_L", "VEL_L", "RHO_L", "P_L", "ACC_L", "STATE_L",
+                      "POS_H", "VEL_H", "RHO_H", "STATE_H", "LIST_L",
+                      "LIST_H", "LIST_G", "DT"):
+                np.testing.assert_array_equal(b.read(sid, k), alone.read(k),
+                                              err_msg=f"{s.name}:{k}")
+            cb, _ = b.conservation(sid)
+            ca, _ = alone.conservation()
+            np.testing.assert_array_equal(cb, ca)
+            alone.close()
+        # retire the middle scene; the others keep advancing unchanged
+        b.remove_scene(ids[1])
+        b.step(3)
+        for s, sid in ((scs[0], ids[0]), (scs[2], ids[2])):
+            alone = _solver(s)
+            alone.step(15)
+            np.testing.assert_array_equal(b.read(sid, "POS_L"),
+                                          alone.read("POS_L"))
+            alone.close()
+    finally:
+        b.close()
+
+
+def test_facade_demo_runs():
+    exe = os.path.join(os.path.dirname(os.path.dirname(__file__)),
+                       "twoscalesplit_amd", "lib", "facade_demo")
+    _say("facade demo (reference class API), 2000 steps")
+    out = subprocess.run([exe, "2000"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    assert res["finite"] == 1 and res["n_low"] == 101 * 151
+    # the collapsing column crosses x = 0.5, where the reference rule splits
+    assert res["n_high"] > 0 and res["n_high"] % 4 == 0
+    assert 900.0 < res["rho_mean"] < 1100.0

@@ -1,0 +1,162 @@
+"""ctypes binding of libtss.so (include/tss.h).
+
+The library is loaded from this package's ``lib/`` directory; a missing or
+unloadable library raises at once -- there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
+                        "libtss.so")
+
+TSS_OK = 0
+MODE_PLAIN, MODE_REFERENCE, MODE_BAND = 0, 1, 2
+DT_FIXED, DT_CFL = 0, 1
+
+FIELDS = [
+    "POS_L", "VEL_L", "ACC_L", "RHO_L", "P_L", "STATE_L", "AGE_L", "VIS_L",
+    "POS_H", "VEL_H", "ACC_H", "RHO_H", "P_H", "STATE_H", "VIS_H",
+    "LIST_L", "LIST_H", "LIST_G", "SORTED_L", "SORTED_H", "KEYS_L", "KEYS_H",
+    "CELL_START_L", "CELL_END_L", "CELL_START_H", "CELL_END_H",
+    "BND_SORTED", "BND_START", "BND_END", "POS_B", "DT",
+]
+FIELD = {name: i for i, name in enumerate(FIELDS)}
+
+
+class TssConfig(C.Structure):
+    _fields_ = [
+        ("domain_origin", C.c_float * 2),
+        ("domain_end", C.c_float * 2),
+        ("domain_dims", C.c_int * 2),
+        ("domain_dims_high", C.c_int * 2),
+        ("effective_radius", C.c_float),
+        ("effective_radius_high", C.c_float),
+        ("rest_density", C.c_float),
+        ("tait_coefficient", C.c_float),
+        ("speed_sound", C.c_float),
+        ("alpha", C.c_float),
+        ("tension_coefficient", C.c_float),
+        ("time_step", C.c_float),
+        ("gravity", C.c_float),
+        ("mass_low", C.c_float),
+        ("mass_high", C.c_float),
+        ("mass_boundary", C.c_float),
+        ("mode", C.c_int),
+        ("split_x", C.c_float),
+        ("surface_ratio", C.c_float),
+        ("surface_ratio_high", C.c_float),
+        ("band_radius", C.c_float),
+        ("merge_radius", C.c_float),
+        ("promote_min_age", C.c_int),
+        ("promote_max_age", C.c_int),
+        ("promote_fluc", C.c_float),
+        ("cap_high", C.c_int),
+        ("dt_mode", C.c_int),
+        ("cfl", C.c_float),
+        ("dt_max", C.c_float),
+    ]
+
+
+class TssCounts(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "n_low", "n_high", "n_groups", "n_low_last", "n_high_last", "n_split",
+        "n_promote", "n_merge", "n_dissolve", "step", "cap_low", "n_bnd",
+        "cells_low",
+        "cells_high")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TssBatchDesc(C.Structure):
+    _fields_ = [("max_scenes", C.c_int), ("cap_low", C.c_longlong),
+                ("cap_bnd", C.c_longlong), ("cells_low", C.c_longlong),
+                ("cells_high", C.c_longlong)]
+
+
+_P = C.c_void_p
+_F = C.c_float
+_I = C.c_int
+_SZ = C.c_size_t
+
+# name -> (restype, argtypes)
+PROTOTYPES = {
+    "tss_config_init": (_I, [C.POINTER(TssConfig), _F, _F, _F, _F, _F, _F, _F,
+                             _F, _F, _F, _F, _F]),
+    "tss_create": (_I, [C.POINTER(TssConfig), _P, _I, _I, _P, _I,
+                        C.POINTER(_P)]),
+    "tss_step": (_I, [_P, _I]),
+    "tss_step_graph": (_I, [_P, _I]),
+    "tss_counts": (_I, [_P, C.POINTER(TssCounts)]),
+    "tss_read": (_I, [_P, _I, _P, _SZ]),
+    "tss_write": (_I, [_P, _I, _P, _SZ]),
+    "tss_set_counts": (_I, [_P, _I, _I, _I]),
+    "tss_device_ptr": (_I, [_P, _I, C.POINTER(_P)]),
+    "tss_synchronize": (_I, [_P]),
+    "tss_conservation": (_I, [_P, _P, _P]),
+    "tss_neighbors": (_I, [_P, _I, _P, _P, _SZ]),
+    "tss_enable_timing": (_I, [_P, _I]),
+    "tss_phase_times": (_I, [_P, _P]),
+    "tss_destroy": (None, [_P]),
+    "tss_batch_memory_bytes": (_I, [C.POINTER(TssBatchDesc),
+                                    C.POINTER(C.c_longlong)]),
+    "tss_batch_create": (_I, [C.POINTER(TssBatchDesc), C.POINTER(_P)]),
+    "tss_batch_add_scene": (_I, [_P, C.POINTER(TssConfig), _P, _I, _I, _P, _I,
+                                 C.POINTER(_I)]),
+    "tss_batch_remove_scene": (_I, [_P, _I]),
+    "tss_batch_step": (_I, [_P, _I]),
+    "tss_batch_step_graph": (_I, [_P, _I]),
+    "tss_batch_counts": (_I, [_P, _I, C.POINTER(TssCounts)]),
+    "tss_batch_read": (_I, [_P, _I, _I, _P, _SZ]),
+    "tss_batch_conservation": (_I, [_P, _I, _P, _P]),
+    "tss_batch_synchronize": (_I, [_P]),
+    "tss_batch_num_scenes": (_I, [_P, C.POINTER(_I)]),
+    "tss_batch_set_tuning": (_I, [_P, _I, _I]),
+    "tss_batch_destroy": (None, [_P]),
+    "tss_canvas_count": (_I, [_I, _I, _I]),
+    "tss_scene_dam_break": (_I, [C.c_uint64, C.c_uint64, _I, _I, _F, _F, _I,
+                                 _I, _I, _P, _P]),
+    "tss_splitmix64_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
+    "tss_last_error": (C.c_char_p, []),
+    "tss_version": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libtss.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libtss.so not built at {LIB_PATH}; run "
+                "`python -m twoscalesplit_amd.build`")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+class TssError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        msg = lib().tss_last_error().decode(errors="replace")
+        super().__init__(f"{what} failed ({code}): {msg}")
+        self.code = code
+
+
+def check(rc: int, what: str) -> None:
+    if rc != TSS_OK:
+        raise TssError(rc, what)
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)

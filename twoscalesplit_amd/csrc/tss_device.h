@@ -1,0 +1,161 @@
+// tss_device.h -- device-side data layout of a batch of two-scale WCSPH
+// scenes and the launch interface of the HIP kernels (tss_kernels.hip).
+//
+// Layout (one "batch" = any number of independent scenes; a single-scene
+// solver is a batch of one):
+//  * Unified particle index space u: low ids [0, CL), high ids [CL, 5 CL);
+//    child k of low id p is u = CL + 4p + k, so a scene that owns low ids
+//    [lbase, lbase + lcap) owns the children [CL + 4 lbase, CL + 4(lbase+lcap)).
+//  * Per-particle state is SoA by u: pos/vel/acc (float2), rho, p, vis
+//    (float), state (u8); age and scene index by low id.
+//  * Unified cell index space: low cells of all scenes first, then high
+//    cells; a scene owns [cbase_l, cbase_l + dlx*dly) and
+//    [cbase_h, cbase_h + dhx*dhy). The sort key of a particle is its unified
+//    cell index, so one stable sort groups particles by level, scene, cell.
+//  * After the sort, the step works on cell-sorted SoA copies (spos, svel,
+//    srho, spr = p/rho^2, sstate, sscene) so every neighbour sum streams
+//    contiguous rows of cells.
+#pragma once
+
+#include <cstdint>
+
+namespace tss {
+
+constexpr int kMaxPasses = 4;     // radix passes (8-bit digits, <= 32-bit keys)
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort tile
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // entries per compaction tile
+constexpr int kRedChunks = 32;    // reduction chunks per scene
+// mass, px, py, ke, pe, max|v|^2, sum|px|, sum|py|, sum|pe|
+constexpr int kRedFields = 9;
+
+struct SceneDev {
+    // geometry
+    float ox, oy;             // domain origin
+    float h, H;               // support radii (low / high)
+    int dlx, dly, dhx, dhy;   // grid dimensions
+    int cbase_l, cbase_h;     // unified cell bases
+    int lbase, lcap;          // low ids owned
+    int bbase, nb;            // boundary sorted range
+    // physics
+    float rho0, B, c, alpha, sigma, g;
+    float m_l, m_h, m_b;
+    float sig_h2, sig_H2;     // M4 coefficients / 2  (W_bar = sum of both)
+    float gc_h2, gc_H2;       // gradient coefficients / 2
+    float q_h, q_H;           // 2/h, 2/H
+    float visc_h, visc_H, visc_x;  // 2*l*alpha*c for l = h, H, (h+H)/2
+    float eps_h, eps_H, eps_x;     // 0.01*l^2
+    float cb_l, cb_h;         // m_b/(m_l+m_b), m_b/(m_h+m_b)
+    float cwall;              // 0.02*c^2
+    // squared-distance thresholds equivalent to the FP32 sqrt predicates
+    float t_le_h, t_lt_h, t_le_H, t_lt_H, t_band, t_merge;
+    float r_band, r_merge;
+    // adaptivity
+    int mode;
+    float split_x;
+    float surf_t;             // surface_ratio * rho0 (FP32 product)
+    float surf_t_h;           // surface_ratio_high * rho0
+    int pmin, pmax;
+    float pfluc;
+    int cap_h;
+    // time step
+    int dt_mode;
+    float cfl, dt_max;
+    float dt;                 // dt of the next step (device-updated)
+    int alive;
+    int pad[2];
+};
+
+struct SceneCount {
+    int n_l, n_h, n_g;        // next-step lists (per scene)
+    int n_l_last, n_h_last;   // lists advanced by the last step
+    int n_split, n_promote, n_merge, n_dissolve;
+    int cand;                 // split candidates of the last step
+    int step;
+    int seg_l0, seg_l1, seg_h0, seg_h1;  // sorted segments of the last step
+    // compaction prefixes at the scene's segment ends (kept low, kept high,
+    // admitted splits): counts = end - begin, no hot atomics
+    int kl0, kl1, kh0, kh1, sp0, sp1;
+};
+
+struct DevCounts {
+    int n_l, n_h, n_g;        // lists the next step reads
+    int n_l_next, n_h_next, n_g_next;
+    int n_l_last, n_h_last;   // lists the last step advanced (sorted sizes)
+};
+
+// Device buffers of a batch (all pointers are device memory).
+struct DevState {
+    int CL;                   // low capacity (unified index space size 5 CL)
+    int NB;                   // boundary capacity
+    int CCL, CCH;             // low / high cell capacity
+    int S;                    // scene capacity
+    SceneDev* scenes;
+    SceneCount* scnt;
+    DevCounts* counts;
+    int* scene_of;            // [CL] scene index of each low id (-1 free)
+    float2* pos;
+    float2* vel;
+    float2* acc;              // [5 CL]
+    float* rho;
+    float* p;
+    float* vis;               // [5 CL]
+    uint8_t* state;           // [5 CL]
+    int* age;                 // [CL]
+    int* list_l[2];           // [CL]   unified ids
+    int* list_h[2];           // [4 CL]
+    int* list_g[2];           // [CL]   low ids of parents
+    uint32_t* keys[2];        // [5 CL] sort ping-pong
+    uint32_t* vals[2];
+    uint32_t* hist;           // [256 * max_tiles]
+    uint32_t* digit_tot;      // [kMaxPasses * 256]
+    int* cstart;
+    int* cend;                // [CCL + CCH]
+    int* bstart;
+    int* bend;                // [CCL]
+    float2* bpos;             // [NB] boundary positions in cell order
+    int* bids;                // [NB] boundary ids (scene-local) in cell order
+    float2* spos;
+    float2* svel;             // [5 CL] cell-sorted copies
+    float* srho;
+    float* spr;               // p / rho^2
+    uint8_t* sstate;
+    uint8_t* ssurf;           // surface flag (sorted)
+    int* sscene;
+    int* sidx;                // [5 CL] sorted index of unified id
+    uint8_t* cand;            // [CL] split candidate flag per sorted low index
+    uint8_t* merged;          // [CL] merged flag per low id
+    int* tile_sums;           // [2 * max_scan_tiles]
+    int* tile_base;           // [2 * max_scan_tiles]
+    int* seg_base;            // [8]
+    double* red;              // [S * kRedChunks * kRedFields]
+    double* cons;             // [S * 10] conservation sums + bounds
+};
+
+// Host-known launch parameters of one step.
+struct StepShape {
+    int passes;               // radix passes
+    int key_bits;
+    int grid_elem;            // blocks of element-wise kernels
+    int sort_tiles;           // max sort tiles (capacity-based)
+    int scan_tiles_l, scan_tiles_h, scan_tiles_g;  // capacity-based tiles
+    int dens_block, force_block;
+    int S;
+    int list_parity;          // which list buffer holds the current lists
+};
+
+// Kernel launchers (tss_kernels.hip). All enqueue on `stream`.
+void launch_step(const DevState& d, const StepShape& sh, void* stream,
+                 void* timing_events /* hipEvent_t[6] or nullptr */);
+void launch_neighbors_n(const DevState& d, const StepShape& sh, int scene,
+                        int level, int n, uint32_t* counts, uint32_t* sums,
+                        void* stream);
+void launch_grid_only(const DevState& d, const StepShape& sh, void* stream);
+void launch_fill_int(int* p, int value, long long n, void* stream);
+void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
+                         void* stream);
+
+}  // namespace tss

@@ -1,0 +1,1232 @@
+// tss_host.cpp -- host runtime of libtss: batches of scenes in one device
+// state, scene admission / retirement, the step driver (eager and hipGraph),
+// field I/O, the seeded scene generator and the C ABI of include/tss.h.
+//
+// Reference: segfault11/TwoScaleSplit TwoScale2D/ ("R/"): WCSPHSolver.{h,cu},
+// ParticleSystem.{h,cpp}.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tss.h"
+#include "tss_device.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct TssError : std::runtime_error {
+    int code;
+    TssError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess)                                                  \
+            throw TssError(TSS_E_HIP, std::string(#expr) + ": " +              \
+                                          hipGetErrorString(e_));              \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return TSS_OK;
+    } catch (const TssError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return TSS_E_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return TSS_E_INVALID;
+    }
+}
+
+void require(bool ok, const std::string& msg, int code = TSS_E_INVALID) {
+    if (!ok) throw TssError(code, msg);
+}
+
+// Largest non-negative float f with sqrtf(f) <= R (inclusive) or < R: the
+// squared-distance threshold equivalent to the FP32 predicate on
+// sqrt(dx*dx + dy*dy) (sqrtf is correctly rounded, hence monotone).
+float sqrt_threshold(float R, bool inclusive) {
+    auto ok = [&](uint32_t bits) {
+        float f;
+        std::memcpy(&f, &bits, 4);
+        float r = std::sqrt(f);
+        return inclusive ? (r <= R) : (r < R);
+    };
+    uint32_t lo = 0, hi = 0x7f800000u;  // ok(lo) holds for R > 0
+    if (!ok(lo)) return -1.0f;
+    while (hi - lo > 1) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (ok(mid)) lo = mid; else hi = mid;
+    }
+    float f;
+    std::memcpy(&f, &lo, 4);
+    return f;
+}
+
+constexpr double kPi = 3.14159265358979323846;  // R/WCSPHSolver.cu:19
+
+// Bulk value (in units of rho0) of the high-level density estimator: the
+// blended kernel (W_h + W_H)/2 summed only over r <= H (R:402-474), i.e.
+// (1 + F)/2 with F the fraction of the 2-D M4 kernel W_h inside radius H.
+// F = M(2H/h)/M(2), M(Q) = int_0^Q f(q) q dq, f the M4 shape. 1 when H = h.
+double high_density_ratio(double s) {
+    auto f = [](double q) {
+        double a = std::max(2.0 - q, 0.0), b = std::max(1.0 - q, 0.0);
+        return a * a * a - 4.0 * b * b * b;
+    };
+    auto M = [&](double Q) {
+        const int n = 20000;  // Simpson; f(q) q is piecewise cubic*linear
+        double hq = Q / n, acc = 0.0;
+        for (int i = 0; i <= n; ++i) {
+            double q = i * hq, w = (i == 0 || i == n) ? 1.0 : (i % 2 ? 4.0 : 2.0);
+            acc += w * f(q) * q;
+        }
+        return acc * hq / 3.0;
+    };
+    double Q = std::min(2.0 * s, 2.0);
+    return 0.5 * (1.0 + M(Q) / M(2.0));
+}
+
+int axis_cell_host(float x, float o, float cell, int dim) {
+    float t = ((x + 0.0f) - o) / cell;
+    if (!(t >= 1.0f)) return 0;
+    if (t >= (float)dim) return dim - 1;
+    return (int)t;
+}
+
+// First-fit allocator over an index range.
+class RangeAlloc {
+public:
+    explicit RangeAlloc(long long cap = 0) { reset(cap); }
+    void reset(long long cap) {
+        free_.clear();
+        if (cap > 0) free_[0] = cap;
+    }
+    bool alloc(long long n, long long& start) {
+        if (n == 0) {
+            start = 0;
+            return true;
+        }
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                start = it->first;
+                long long rest = it->second - n;
+                free_.erase(it);
+                if (rest > 0) free_[start + n] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(long long start, long long n) {
+        if (n == 0) return;
+        auto it = free_.emplace(start, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+private:
+    std::map<long long, long long> free_;
+};
+
+struct SceneHost {
+    bool alive = false;
+    tss_config cfg{};
+    int lbase = 0, lcap = 0;
+    int bbase = 0, nb = 0;
+    int cbl = 0, ncl = 0;     // low cells: unified [cbl, cbl + ncl)
+    int cbh = 0, nch = 0;     // high cells: unified [cbh, cbh + nch)
+};
+
+template <class T>
+T* dalloc(size_t n, std::vector<void*>& owned) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw TssError(TSS_E_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) +
+                                        " bytes failed: " + hipGetErrorString(e));
+    }
+    owned.push_back(p);
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+struct tss_batch {
+    tss::DevState d{};
+    tss::StepShape sh{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    std::vector<SceneHost> scenes;
+    RangeAlloc a_low, a_bnd, a_cl, a_ch;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool timing = false;
+    hipEvent_t ev[6] = {};
+    float phase_ms[5] = {0, 0, 0, 0, 0};
+    int dens_block = 256, force_block = 256;
+    long long cap_l = 0, cap_b = 0, ccl = 0, cch = 0;
+    int max_scenes = 0;
+    uint32_t* nb_counts = nullptr;  // neighbour inspection scratch
+    uint32_t* nb_sums = nullptr;
+    size_t nb_cap = 0;
+
+    ~tss_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (void* p : owned) (void)hipFree(p);
+        if (nb_counts) (void)hipFree(nb_counts);
+        if (nb_sums) (void)hipFree(nb_sums);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    static long long bytes_for(const tss_batch_desc& ds) {
+        const long long CL = ds.cap_low, U = 5 * CL;
+        const long long tiles = (U + tss::kSortTile - 1) / tss::kSortTile + 1;
+        long long b = 0;
+        b += U * (8 * 3 + 4 * 3 + 1);             // pos vel acc rho p vis state
+        b += CL * (4 + 4);                         // age scene_of
+        b += 2 * (CL + 4 * CL + CL) * 4;           // lists
+        b += 4 * U * 4;                            // keys/vals x2
+        b += 256 * tiles * 4 + tss::kMaxPasses * 256 * 4;
+        b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 8;
+        b += ds.cap_bnd * 12;
+        b += U * (8 + 8 + 4 + 4 + 1 + 1 + 4 + 4);  // sorted copies + sidx
+        b += CL * 2;
+        b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) +
+                                         (tss::kRedChunks * tss::kRedFields + 10) * 8);
+        return b;
+    }
+
+    void create(const tss_batch_desc& ds) {
+        require(ds.max_scenes >= 1 && ds.cap_low >= 1 && ds.cap_bnd >= 0 &&
+                    ds.cells_low >= 1 && ds.cells_high >= 1,
+                "invalid batch capacities");
+        require(5 * ds.cap_low < (1LL << 31) && ds.cells_low + ds.cells_high < (1LL << 31),
+                "batch capacities exceed 32-bit indexing");
+        cap_l = ds.cap_low;
+        cap_b = ds.cap_bnd;
+        ccl = ds.cells_low;
+        cch = ds.cells_high;
+        max_scenes = ds.max_scenes;
+        HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
+        const long long CL = cap_l, U = 5 * CL;
+        d.CL = (int)CL;
+        d.NB = (int)cap_b;
+        d.CCL = (int)ccl;
+        d.CCH = (int)cch;
+        d.S = max_scenes;
+        d.scenes = dalloc<tss::SceneDev>(max_scenes, owned);
+        d.scnt = dalloc<tss::SceneCount>(max_scenes, owned);
+        d.counts = dalloc<tss::DevCounts>(1, owned);
+        d.scene_of = dalloc<int>(CL, owned);
+        d.pos = dalloc<float2>(U, owned);
+        d.vel = dalloc<float2>(U, owned);
+        d.acc = dalloc<float2>(U, owned);
+        d.rho = dalloc<float>(U, owned);
+        d.p = dalloc<float>(U, owned);
+        d.vis = dalloc<float>(U, owned);
+        d.state = dalloc<uint8_t>(U, owned);
+        d.age = dalloc<int>(CL, owned);
+        for (int i = 0; i < 2; ++i) {
+            d.list_l[i] = dalloc<int>(CL, owned);
+            d.list_h[i] = dalloc<int>(4 * CL, owned);
+            d.list_g[i] = dalloc<int>(CL, owned);
+            d.keys[i] = dalloc<uint32_t>(U, owned);
+            d.vals[i] = dalloc<uint32_t>(U, owned);
+        }
+        sh.sort_tiles = (int)((U + tss::kSortTile - 1) / tss::kSortTile);
+        d.hist = dalloc<uint32_t>((size_t)256 * sh.sort_tiles, owned);
+        d.digit_tot = dalloc<uint32_t>(tss::kMaxPasses * 256, owned);
+        d.cstart = dalloc<int>(ccl + cch, owned);
+        d.cend = dalloc<int>(ccl + cch, owned);
+        d.bstart = dalloc<int>(ccl, owned);
+        d.bend = dalloc<int>(ccl, owned);
+        d.bpos = dalloc<float2>(cap_b, owned);
+        d.bids = dalloc<int>(cap_b, owned);
+        d.spos = dalloc<float2>(U, owned);
+        d.svel = dalloc<float2>(U, owned);
+        d.srho = dalloc<float>(U, owned);
+        d.spr = dalloc<float>(U, owned);
+        d.sstate = dalloc<uint8_t>(U, owned);
+        d.ssurf = dalloc<uint8_t>(U, owned);
+        d.sscene = dalloc<int>(U, owned);
+        d.sidx = dalloc<int>(U, owned);
+        d.cand = dalloc<uint8_t>(CL, owned);
+        d.merged = dalloc<uint8_t>(CL, owned);
+        sh.scan_tiles_l = (int)((CL + tss::kScanTile - 1) / tss::kScanTile);
+        sh.scan_tiles_h = (int)((4 * CL + tss::kScanTile - 1) / tss::kScanTile);
+        sh.scan_tiles_g = sh.scan_tiles_l;
+        const int nst = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
+        d.tile_sums = dalloc<int>(2 * (size_t)nst, owned);
+        d.tile_base = dalloc<int>(2 * (size_t)nst, owned);
+        d.seg_base = dalloc<int>(8, owned);
+        d.red = dalloc<double>((size_t)max_scenes * tss::kRedChunks * tss::kRedFields, owned);
+        d.cons = dalloc<double>((size_t)max_scenes * 10, owned);
+        // initial contents
+        HIP_CHECK(hipMemsetAsync(d.scenes, 0, sizeof(tss::SceneDev) * max_scenes, stream));
+        HIP_CHECK(hipMemsetAsync(d.scnt, 0, sizeof(tss::SceneCount) * max_scenes, stream));
+        HIP_CHECK(hipMemsetAsync(d.counts, 0, sizeof(tss::DevCounts), stream));
+        HIP_CHECK(hipMemsetAsync(d.cons, 0, sizeof(double) * 10 * max_scenes, stream));
+        HIP_CHECK(hipMemsetAsync(d.red, 0, sizeof(double) * tss::kRedChunks * tss::kRedFields * max_scenes, stream));
+        tss::launch_fill_int(d.scene_of, 0, CL, stream);
+        HIP_CHECK(hipMemsetAsync(d.state, 0, (size_t)U, stream));
+        HIP_CHECK(hipMemsetAsync(d.pos, 0, sizeof(float2) * U, stream));
+        HIP_CHECK(hipMemsetAsync(d.vel, 0, sizeof(float2) * U, stream));
+        HIP_CHECK(hipMemsetAsync(d.acc, 0, sizeof(float2) * U, stream));
+        HIP_CHECK(hipMemsetAsync(d.rho, 0, sizeof(float) * U, stream));
+        HIP_CHECK(hipMemsetAsync(d.p, 0, sizeof(float) * U, stream));
+        HIP_CHECK(hipMemsetAsync(d.vis, 0, sizeof(float) * U, stream));
+        HIP_CHECK(hipMemsetAsync(d.age, 0, sizeof(int) * CL, stream));
+        HIP_CHECK(hipMemsetAsync(d.merged, 0, (size_t)CL, stream));
+        HIP_CHECK(hipMemsetAsync(d.cand, 0, (size_t)CL, stream));
+        tss::launch_fill_int(d.cstart, -1, ccl + cch, stream);
+        tss::launch_fill_int(d.cend, -1, ccl + cch, stream);
+        tss::launch_fill_int(d.bstart, -1, ccl, stream);
+        tss::launch_fill_int(d.bend, -1, ccl, stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        a_low.reset(cap_l);
+        a_bnd.reset(cap_b);
+        a_cl.reset(ccl);
+        a_ch.reset(cch);
+        scenes.assign(max_scenes, SceneHost{});
+        sh.grid_elem = (int)std::max<long long>(1, (U + 255) / 256);
+        sh.dens_block = dens_block;
+        sh.force_block = force_block;
+        sh.S = max_scenes;
+        sh.list_parity = 0;
+        update_shape();
+    }
+
+    void update_shape() {
+        long long maxkey = 1;
+        for (auto& s : scenes)
+            if (s.alive) maxkey = std::max<long long>(maxkey, (long long)s.cbh + s.nch);
+        int bits = 1;
+        while ((1LL << bits) < maxkey) ++bits;
+        sh.key_bits = bits;
+        sh.passes = (bits + 7) / 8;
+        sh.dens_block = dens_block;
+        sh.force_block = force_block;
+        invalidate_graphs();
+    }
+
+    void invalidate_graphs() {
+        for (auto& g : graph) {
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        }
+    }
+
+    tss::DevCounts read_counts() {
+        tss::DevCounts c;
+        HIP_CHECK(hipMemcpyAsync(&c, d.counts, sizeof(c), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        return c;
+    }
+
+    void write_counts(const tss::DevCounts& c) {
+        HIP_CHECK(hipMemcpyAsync(d.counts, &c, sizeof(c), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    tss::SceneCount read_scnt(int s) {
+        tss::SceneCount c;
+        HIP_CHECK(hipMemcpyAsync(&c, d.scnt + s, sizeof(c), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        return c;
+    }
+
+    static tss::SceneDev make_scene_dev(const tss_config& c) {
+        tss::SceneDev s{};
+        s.ox = c.domain_origin[0];
+        s.oy = c.domain_origin[1];
+        s.h = c.effective_radius;
+        s.H = c.effective_radius_high;
+        s.dlx = c.domain_dims[0];
+        s.dly = c.domain_dims[1];
+        s.dhx = c.domain_dims_high[0];
+        s.dhy = c.domain_dims_high[1];
+        s.rho0 = c.rest_density;
+        s.B = c.tait_coefficient;
+        s.c = c.speed_sound;
+        s.alpha = c.alpha;
+        s.sigma = c.tension_coefficient;
+        s.g = c.gravity;
+        s.m_l = c.mass_low;
+        s.m_h = c.mass_high;
+        s.m_b = c.mass_boundary;
+        // M4 coefficients with double pi, stored as float (R:2559-2590)
+        const double h = s.h, H = s.H;
+        s.sig_h2 = (float)(10.0 / (7.0 * kPi * h * h) / 2.0);
+        s.sig_H2 = (float)(10.0 / (7.0 * kPi * H * H) / 2.0);
+        s.gc_h2 = (float)(-60.0 / (7.0 * kPi * h * h * h) / 2.0);
+        s.gc_H2 = (float)(-60.0 / (7.0 * kPi * H * H * H) / 2.0);
+        s.q_h = (float)(2.0 / h);
+        s.q_H = (float)(2.0 / H);
+        const float hb = (s.h + s.H) / 2.0f;  // R:623
+        s.visc_h = 2.0f * s.h * s.alpha * s.c;
+        s.visc_H = 2.0f * s.H * s.alpha * s.c;
+        s.visc_x = 2.0f * hb * s.alpha * s.c;
+        s.eps_h = 0.01f * s.h * s.h;
+        s.eps_H = 0.01f * s.H * s.H;
+        s.eps_x = 0.01f * hb * hb;
+        s.cb_l = s.m_b / (s.m_l + s.m_b);
+        s.cb_h = s.m_b / (s.m_h + s.m_b);
+        s.cwall = 0.02f * s.c * s.c;
+        s.t_le_h = sqrt_threshold(s.h, true);
+        s.t_lt_h = sqrt_threshold(s.h, false);
+        s.t_le_H = sqrt_threshold(s.H, true);
+        s.t_lt_H = sqrt_threshold(s.H, false);
+        s.r_band = c.band_radius;
+        s.r_merge = c.merge_radius;
+        s.t_band = sqrt_threshold(c.band_radius, true);
+        s.t_merge = sqrt_threshold(c.merge_radius, true);
+        s.mode = c.mode;
+        s.split_x = c.split_x;
+        s.surf_t = c.surface_ratio * c.rest_density;
+        s.surf_t_h = c.surface_ratio_high * c.rest_density;
+        s.pmin = c.promote_min_age;
+        s.pmax = c.promote_max_age;
+        s.pfluc = c.promote_fluc;
+        s.cap_h = c.cap_high;
+        s.dt_mode = c.dt_mode;
+        s.cfl = c.cfl;
+        s.dt_max = c.dt_max;
+        s.dt = c.time_step;
+        s.alive = 1;
+        return s;
+    }
+
+    static void validate(const tss_config& c, int n_low, int cap_low, int n_bnd) {
+        require(c.domain_origin[0] < c.domain_end[0] && c.domain_origin[1] < c.domain_end[1],
+                "Invalid configuration parameters: empty domain");
+        require(c.effective_radius > 0 && c.effective_radius_high > 0 &&
+                    c.effective_radius_high <= c.effective_radius,
+                "Invalid configuration parameters: radii");
+        require(c.domain_dims[0] >= 1 && c.domain_dims[1] >= 1 && c.domain_dims_high[0] >= 1 &&
+                    c.domain_dims_high[1] >= 1,
+                "Invalid configuration parameters: grid dimensions (call tss_config_init)");
+        require((long long)c.domain_dims_high[0] * c.domain_dims_high[1] < (1LL << 30),
+                "grid too large");
+        require(n_low >= 0 && cap_low >= n_low && cap_low >= 1, "invalid particle counts");
+        require(n_bnd >= 0, "invalid boundary count");
+        require(c.mode >= 0 && c.mode <= 2, "invalid mode");
+        require(c.cap_high >= 0 && (long long)c.cap_high <= 4LL * cap_low,
+                "cap_high must be within 0..4*cap_low (child slots 4p+k)");
+        require(c.band_radius > 0 && c.merge_radius > 0, "band/merge radius must be > 0");
+        require(c.rest_density > 0 && c.mass_low > 0 && c.mass_high > 0, "masses/density must be > 0");
+        require(c.dt_mode == TSS_DT_FIXED || (c.dt_mode == TSS_DT_CFL && c.cfl > 0 && c.dt_max > 0),
+                "invalid time-step control");
+        require(c.time_step > 0, "time step must be > 0");
+    }
+
+    int add_scene(const tss_config& c, const float* low_xy, int n_low, int cap_low,
+                  const float* bnd_xy, int n_bnd) {
+        validate(c, n_low, cap_low, n_bnd);
+        for (int i = 0; i < 2 * n_low; ++i)
+            require(std::isfinite(low_xy[i]), "non-finite particle position");
+        for (int i = 0; i < 2 * n_bnd; ++i)
+            require(std::isfinite(bnd_xy[i]), "non-finite boundary position");
+        int sid = -1;
+        for (int i = 0; i < max_scenes; ++i)
+            if (!scenes[i].alive) {
+                sid = i;
+                break;
+            }
+        require(sid >= 0, "batch holds max_scenes scenes", TSS_E_NOMEM);
+        const long long ncl = (long long)c.domain_dims[0] * c.domain_dims[1];
+        const long long nch = (long long)c.domain_dims_high[0] * c.domain_dims_high[1];
+        long long lb, bb, cl, ch;
+        if (!a_low.alloc(cap_low, lb)) throw TssError(TSS_E_NOMEM, "low-particle capacity exhausted");
+        if (!a_bnd.alloc(n_bnd, bb)) {
+            a_low.release(lb, cap_low);
+            throw TssError(TSS_E_NOMEM, "boundary capacity exhausted");
+        }
+        if (!a_cl.alloc(ncl, cl)) {
+            a_low.release(lb, cap_low);
+            a_bnd.release(bb, n_bnd);
+            throw TssError(TSS_E_NOMEM, "low-cell capacity exhausted");
+        }
+        if (!a_ch.alloc(nch, ch)) {
+            a_low.release(lb, cap_low);
+            a_bnd.release(bb, n_bnd);
+            a_cl.release(cl, ncl);
+            throw TssError(TSS_E_NOMEM, "high-cell capacity exhausted");
+        }
+        SceneHost& sh_ = scenes[sid];
+        sh_.cfg = c;
+        sh_.lbase = (int)lb;
+        sh_.lcap = cap_low;
+        sh_.bbase = (int)bb;
+        sh_.nb = n_bnd;
+        sh_.cbl = (int)cl;
+        sh_.ncl = (int)ncl;
+        sh_.cbh = (int)(ccl + ch);
+        sh_.nch = (int)nch;
+        tss::SceneDev sd = make_scene_dev(c);
+        sd.cbase_l = sh_.cbl;
+        sd.cbase_h = sh_.cbh;
+        sd.lbase = sh_.lbase;
+        sd.lcap = cap_low;
+        sd.bbase = sh_.bbase;
+        sd.nb = n_bnd;
+
+        tss::DevCounts cnt = read_counts();
+        const int CL = d.CL;
+        const int lbase = sh_.lbase;
+        // particle state
+        HIP_CHECK(hipMemcpyAsync(d.pos + lbase, low_xy, sizeof(float) * 2 * (size_t)n_low,
+                                 hipMemcpyHostToDevice, stream));
+        if (cap_low > n_low)
+            HIP_CHECK(hipMemsetAsync(d.pos + lbase + n_low, 0, sizeof(float2) * (size_t)(cap_low - n_low), stream));
+        HIP_CHECK(hipMemsetAsync(d.vel + lbase, 0, sizeof(float2) * (size_t)cap_low, stream));
+        HIP_CHECK(hipMemsetAsync(d.acc + lbase, 0, sizeof(float2) * (size_t)cap_low, stream));
+        HIP_CHECK(hipMemsetAsync(d.rho + lbase, 0, sizeof(float) * (size_t)cap_low, stream));
+        HIP_CHECK(hipMemsetAsync(d.p + lbase, 0, sizeof(float) * (size_t)cap_low, stream));
+        HIP_CHECK(hipMemsetAsync(d.vis + lbase, 0, sizeof(float) * (size_t)cap_low, stream));
+        HIP_CHECK(hipMemsetAsync(d.state + lbase, 0, (size_t)cap_low, stream));
+        HIP_CHECK(hipMemsetAsync(d.age + lbase, 0, sizeof(int) * (size_t)cap_low, stream));
+        const size_t hb = (size_t)CL + 4 * (size_t)lbase, hn = 4 * (size_t)cap_low;
+        HIP_CHECK(hipMemsetAsync(d.pos + hb, 0, sizeof(float2) * hn, stream));
+        HIP_CHECK(hipMemsetAsync(d.vel + hb, 0, sizeof(float2) * hn, stream));
+        HIP_CHECK(hipMemsetAsync(d.acc + hb, 0, sizeof(float2) * hn, stream));
+        HIP_CHECK(hipMemsetAsync(d.rho + hb, 0, sizeof(float) * hn, stream));
+        HIP_CHECK(hipMemsetAsync(d.p + hb, 0, sizeof(float) * hn, stream));
+        HIP_CHECK(hipMemsetAsync(d.vis + hb, 0, sizeof(float) * hn, stream));
+        HIP_CHECK(hipMemsetAsync(d.state + hb, 1, hn, stream));
+        tss::launch_fill_int(d.scene_of + lbase, sid, cap_low, stream);
+        // boundary grid on the low grid (initBoundaries, R:3205-3238): iota
+        // ids, hash, stable sort, cell ranges
+        std::vector<int> bh(n_bnd), order(n_bnd);
+        for (int i = 0; i < n_bnd; ++i) {
+            int cx = axis_cell_host(bnd_xy[2 * i], c.domain_origin[0], c.effective_radius, c.domain_dims[0]);
+            int cy = axis_cell_host(bnd_xy[2 * i + 1], c.domain_origin[1], c.effective_radius, c.domain_dims[1]);
+            bh[i] = cy * c.domain_dims[0] + cx;
+        }
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bh[a] < bh[b]; });
+        std::vector<float2> bpos(n_bnd);
+        std::vector<int> bids(n_bnd);
+        std::vector<int> bst(ncl, -1), ben(ncl, -1);
+        for (int k = 0; k < n_bnd; ++k) {
+            int i = order[k];
+            bpos[k] = make_float2(bnd_xy[2 * i], bnd_xy[2 * i + 1]);
+            bids[k] = i;
+            int key = bh[i];
+            if (k == 0 || bh[order[k - 1]] != key) bst[key] = (int)bb + k;
+            if (k == n_bnd - 1 || bh[order[k + 1]] != key) ben[key] = (int)bb + k + 1;
+        }
+        if (n_bnd) {
+            HIP_CHECK(hipMemcpyAsync(d.bpos + bb, bpos.data(), sizeof(float2) * n_bnd, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipMemcpyAsync(d.bids + bb, bids.data(), sizeof(int) * n_bnd, hipMemcpyHostToDevice, stream));
+        }
+        HIP_CHECK(hipMemcpyAsync(d.bstart + cl, bst.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d.bend + cl, ben.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, stream));
+        // initial low list: ids in lattice order (the reference's iota list,
+        // R/WCSPHSolver.cu:2358-2371), appended to the current list
+        std::vector<int> ids(n_low);
+        for (int i = 0; i < n_low; ++i) ids[i] = lbase + i;
+        const int par = sh.list_parity;
+        if (n_low)
+            HIP_CHECK(hipMemcpyAsync(d.list_l[par] + cnt.n_l, ids.data(), sizeof(int) * n_low,
+                                     hipMemcpyHostToDevice, stream));
+        cnt.n_l += n_low;
+        tss::SceneCount sc{};
+        sc.n_l = n_low;
+        HIP_CHECK(hipMemcpyAsync(d.scnt + sid, &sc, sizeof(sc), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d.scenes + sid, &sd, sizeof(sd), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemsetAsync(d.cons + (size_t)sid * 10, 0, sizeof(double) * 10, stream));
+        write_counts(cnt);
+        sh_.alive = true;
+        update_shape();
+        return sid;
+    }
+
+    void remove_scene(int sid) {
+        require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
+        SceneHost& s = scenes[sid];
+        tss::launch_filter_scene(d, sh, sid, stream);
+        tss::SceneDev sd{};
+        HIP_CHECK(hipMemcpyAsync(d.scenes + sid, &sd, sizeof(sd), hipMemcpyHostToDevice, stream));
+        tss::launch_fill_int(d.bstart + s.cbl, -1, s.ncl, stream);
+        tss::launch_fill_int(d.bend + s.cbl, -1, s.ncl, stream);
+        HIP_CHECK(hipStreamSynchronize(stream));
+        a_low.release(s.lbase, s.lcap);
+        a_bnd.release(s.bbase, s.nb);
+        a_cl.release(s.cbl, s.ncl);
+        a_ch.release(s.cbh - ccl, s.nch);
+        s = SceneHost{};
+        update_shape();
+    }
+
+    void step_eager(int n) {
+        for (int i = 0; i < n; ++i) {
+            tss::launch_step(d, sh, stream, timing ? (void*)ev : nullptr);
+            sh.list_parity ^= 1;
+        }
+        if (timing && n > 0) {
+            HIP_CHECK(hipEventSynchronize(ev[5]));
+            float t;
+            HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
+            phase_ms[0] = t;
+            HIP_CHECK(hipEventElapsedTime(&t, ev[1], ev[2]));
+            phase_ms[1] = t;
+            HIP_CHECK(hipEventElapsedTime(&t, ev[2], ev[3]));
+            phase_ms[2] = t;
+            HIP_CHECK(hipEventElapsedTime(&t, ev[3], ev[4]));
+            phase_ms[3] = t;
+            HIP_CHECK(hipEventElapsedTime(&t, ev[4], ev[5]));
+            phase_ms[4] = t;
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // Two consecutive steps captured once per starting list parity and
+    // replayed: no host synchronisation, counts stay on the device.
+    void step_graph(int n) {
+        const int pairs = n / 2;
+        if (pairs > 0) {
+            const int par = sh.list_parity;
+            if (!graph[par]) {
+                hipGraph_t g;
+                HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                tss::StepShape s2 = sh;
+                tss::launch_step(d, s2, stream, nullptr);
+                s2.list_parity ^= 1;
+                tss::launch_step(d, s2, stream, nullptr);
+                HIP_CHECK(hipStreamEndCapture(stream, &g));
+                hipError_t e = hipGraphInstantiate(&graph[par], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                HIP_CHECK(e);
+            }
+            for (int i = 0; i < pairs; ++i) HIP_CHECK(hipGraphLaunch(graph[par], stream));
+        }
+        if (n & 1) step_eager(1);
+        HIP_CHECK(hipGetLastError());
+    }
+
+    // ---- field I/O -------------------------------------------------------
+    struct FieldView {
+        void* dev;          // device base of the scene's slice
+        size_t elem;        // bytes per element
+        size_t count;       // elements
+        int kind;           // 0 plain, 1 low-id list, 2 high-id list, 3 sorted-low, 4 sorted-high,
+                            // 5 keys-low, 6 keys-high, 7 cells-low (rel seg_l0), 8 cells-high (rel seg_h0),
+                            // 9 bnd ranges (rel bbase), 10 dt
+    };
+
+    FieldView view(int sid, int field) {
+        require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
+        const SceneHost& s = scenes[sid];
+        const size_t lb = s.lbase, lc = s.lcap;
+        const size_t hb = (size_t)d.CL + 4 * lb, hc = 4 * lc;
+        switch (field) {
+            case TSS_POS_L: return {d.pos + lb, 8, lc, 0};
+            case TSS_VEL_L: return {d.vel + lb, 8, lc, 0};
+            case TSS_ACC_L: return {d.acc + lb, 8, lc, 0};
+            case TSS_RHO_L: return {d.rho + lb, 4, lc, 0};
+            case TSS_P_L: return {d.p + lb, 4, lc, 0};
+            case TSS_STATE_L: return {d.state + lb, 1, lc, 0};
+            case TSS_AGE_L: return {d.age + lb, 4, lc, 0};
+            case TSS_VIS_L: return {d.vis + lb, 4, lc, 0};
+            case TSS_POS_H: return {d.pos + hb, 8, hc, 0};
+            case TSS_VEL_H: return {d.vel + hb, 8, hc, 0};
+            case TSS_ACC_H: return {d.acc + hb, 8, hc, 0};
+            case TSS_RHO_H: return {d.rho + hb, 4, hc, 0};
+            case TSS_P_H: return {d.p + hb, 4, hc, 0};
+            case TSS_STATE_H: return {d.state + hb, 1, hc, 0};
+            case TSS_VIS_H: return {d.vis + hb, 4, hc, 0};
+            case TSS_LIST_L: return {d.list_l[sh.list_parity], 4, 0, 1};
+            case TSS_LIST_H: return {d.list_h[sh.list_parity], 4, 0, 2};
+            case TSS_LIST_G: return {d.list_g[sh.list_parity], 4, 0, 11};
+            case TSS_SORTED_L: return {d.vals[sh.passes & 1], 4, 0, 3};
+            case TSS_SORTED_H: return {d.vals[sh.passes & 1], 4, 0, 4};
+            case TSS_KEYS_L: return {d.keys[sh.passes & 1], 4, 0, 5};
+            case TSS_KEYS_H: return {d.keys[sh.passes & 1], 4, 0, 6};
+            case TSS_CELL_START_L: return {d.cstart + s.cbl, 4, (size_t)s.ncl, 7};
+            case TSS_CELL_END_L: return {d.cend + s.cbl, 4, (size_t)s.ncl, 7};
+            case TSS_CELL_START_H: return {d.cstart + s.cbh, 4, (size_t)s.nch, 8};
+            case TSS_CELL_END_H: return {d.cend + s.cbh, 4, (size_t)s.nch, 8};
+            case TSS_BND_SORTED: return {d.bids + s.bbase, 4, (size_t)s.nb, 0};
+            case TSS_BND_START: return {d.bstart + s.cbl, 4, (size_t)s.ncl, 9};
+            case TSS_BND_END: return {d.bend + s.cbl, 4, (size_t)s.ncl, 9};
+            case TSS_POS_B: return {nullptr, 8, (size_t)s.nb, 12};
+            case TSS_DT: return {&d.scenes[sid].dt, 4, 1, 0};
+            default: break;
+        }
+        throw TssError(TSS_E_INVALID, "unknown field " + std::to_string(field));
+    }
+
+    void read_field(int sid, int field, void* dst, size_t bytes) {
+        FieldView v = view(sid, field);
+        const SceneHost& s = scenes[sid];
+        const int CL = d.CL;
+        if (v.kind == 0) {
+            require(bytes <= v.elem * v.count, "read exceeds field size");
+            HIP_CHECK(hipMemcpyAsync(dst, v.dev, bytes, hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            return;
+        }
+        if (v.kind == 12) {
+            // boundary positions by id, from the cell-ordered copy
+            std::vector<float2> bp(s.nb);
+            std::vector<int> bi(s.nb);
+            if (s.nb) {
+                HIP_CHECK(hipMemcpyAsync(bp.data(), d.bpos + s.bbase, 8 * (size_t)s.nb, hipMemcpyDeviceToHost, stream));
+                HIP_CHECK(hipMemcpyAsync(bi.data(), d.bids + s.bbase, 4 * (size_t)s.nb, hipMemcpyDeviceToHost, stream));
+            }
+            HIP_CHECK(hipStreamSynchronize(stream));
+            std::vector<float2> out(s.nb);
+            for (int k = 0; k < s.nb; ++k) out[bi[k]] = bp[k];
+            require(bytes <= 8 * (size_t)s.nb, "read exceeds field size");
+            std::memcpy(dst, out.data(), bytes);
+            return;
+        }
+        tss::DevCounts c = read_counts();
+        tss::SceneCount sc = read_scnt(sid);
+        std::vector<int> buf;
+        auto fetch = [&](const void* src, size_t off, size_t n) {
+            buf.resize(n);
+            if (n) {
+                HIP_CHECK(hipMemcpyAsync(buf.data(), (const int*)src + off, 4 * n, hipMemcpyDeviceToHost, stream));
+                HIP_CHECK(hipStreamSynchronize(stream));
+            }
+        };
+        std::vector<int> out;
+        switch (v.kind) {
+            case 1:  // current low list, this scene's ids (local)
+                fetch(v.dev, 0, c.n_l);
+                for (int u : buf)
+                    if (u >= s.lbase && u < s.lbase + s.lcap) out.push_back(u - s.lbase);
+                break;
+            case 2:
+                fetch(v.dev, 0, c.n_h);
+                for (int u : buf) {
+                    int a = u - CL;
+                    if (a >= 4 * s.lbase && a < 4 * (s.lbase + s.lcap)) out.push_back(a - 4 * s.lbase);
+                }
+                break;
+            case 11:
+                fetch(v.dev, 0, c.n_g);
+                for (int p : buf)
+                    if (p >= s.lbase && p < s.lbase + s.lcap) out.push_back(p - s.lbase);
+                break;
+            case 3:
+                fetch(v.dev, sc.seg_l0, sc.seg_l1 - sc.seg_l0);
+                for (int u : buf) out.push_back(u - s.lbase);
+                break;
+            case 4:
+                fetch(v.dev, sc.seg_h0, sc.seg_h1 - sc.seg_h0);
+                for (int u : buf) out.push_back(u - CL - 4 * s.lbase);
+                break;
+            case 5:
+                fetch(v.dev, sc.seg_l0, sc.seg_l1 - sc.seg_l0);
+                for (int k : buf) out.push_back(k - s.cbl);
+                break;
+            case 6:
+                fetch(v.dev, sc.seg_h0, sc.seg_h1 - sc.seg_h0);
+                for (int k : buf) out.push_back(k - s.cbh);
+                break;
+            case 7:
+                fetch(v.dev, 0, v.count);
+                for (int x : buf) out.push_back(x < 0 ? -1 : x - sc.seg_l0);
+                break;
+            case 8:
+                fetch(v.dev, 0, v.count);
+                for (int x : buf) out.push_back(x < 0 ? -1 : x - sc.seg_h0);
+                break;
+            case 9:
+                fetch(v.dev, 0, v.count);
+                for (int x : buf) out.push_back(x < 0 ? -1 : x - s.bbase);
+                break;
+            default:
+                throw TssError(TSS_E_INVALID, "field not readable");
+        }
+        require(bytes <= out.size() * 4, "read exceeds field size (" + std::to_string(out.size()) + " entries)");
+        std::memcpy(dst, out.data(), bytes);
+    }
+
+    void write_field(int sid, int field, const void* src, size_t bytes) {
+        FieldView v = view(sid, field);
+        const SceneHost& s = scenes[sid];
+        if (v.kind == 0 && field != TSS_DT) {
+            require(bytes <= v.elem * v.count, "write exceeds field size");
+            HIP_CHECK(hipMemcpyAsync(v.dev, src, bytes, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            invalidate_graphs();
+            return;
+        }
+        if (field == TSS_DT) {
+            require(bytes == 4, "dt is one float");
+            HIP_CHECK(hipMemcpyAsync(v.dev, src, 4, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            return;
+        }
+        // lists: only for single-scene batches (re-seeding a state)
+        int alive = 0;
+        for (auto& x : scenes) alive += x.alive;
+        require(alive == 1, "list writes need a batch of one scene");
+        const int n = (int)(bytes / 4);
+        std::vector<int> buf(n);
+        std::memcpy(buf.data(), src, bytes);
+        if (v.kind == 1 || v.kind == 11) {
+            require(n <= s.lcap, "list too long");
+            for (int& x : buf) {
+                require(x >= 0 && x < s.lcap, "list id out of range");
+                x += s.lbase;
+            }
+        } else if (v.kind == 2) {
+            require(n <= 4 * s.lcap, "list too long");
+            for (int& x : buf) {
+                require(x >= 0 && x < 4 * s.lcap, "list id out of range");
+                x += d.CL + 4 * s.lbase;
+            }
+        } else {
+            throw TssError(TSS_E_INVALID, "field not writable");
+        }
+        if (n) {
+            HIP_CHECK(hipMemcpyAsync(v.dev, buf.data(), bytes, hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+
+    void set_counts(int sid, int nl, int nh, int ng) {
+        int alive = 0;
+        for (auto& x : scenes) alive += x.alive;
+        require(alive == 1 && scenes[sid].alive, "set_counts needs a batch of one scene");
+        const SceneHost& s = scenes[sid];
+        require(nl >= 0 && nl <= s.lcap && nh >= 0 && nh <= 4 * s.lcap && ng >= 0 && ng <= s.lcap,
+                "counts out of range");
+        tss::DevCounts c = read_counts();
+        c.n_l = nl;
+        c.n_h = nh;
+        c.n_g = ng;
+        write_counts(c);
+        tss::SceneCount sc = read_scnt(sid);
+        sc.n_l = nl;
+        sc.n_h = nh;
+        sc.n_g = ng;
+        HIP_CHECK(hipMemcpyAsync(d.scnt + sid, &sc, sizeof(sc), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void counts(int sid, tss_counts_t* o) {
+        require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
+        tss::SceneCount sc = read_scnt(sid);
+        const SceneHost& s = scenes[sid];
+        o->n_low = sc.n_l;
+        o->n_high = sc.n_h;
+        o->n_groups = sc.n_g;
+        o->n_low_last = sc.n_l_last;
+        o->n_high_last = sc.n_h_last;
+        o->n_split = sc.n_split;
+        o->n_promote = sc.n_promote;
+        o->n_merge = sc.n_merge;
+        o->n_dissolve = sc.n_dissolve;
+        o->step = sc.step;
+        o->cap_low = s.lcap;
+        o->n_bnd = s.nb;
+        o->cells_low = s.ncl;
+        o->cells_high = s.nch;
+    }
+
+    void conservation(int sid, double* out, double* err) {
+        require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
+        double buf[10];
+        HIP_CHECK(hipMemcpyAsync(buf, d.cons + (size_t)sid * 10, sizeof(buf), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (int i = 0; i < 5; ++i) {
+            out[i] = buf[i];
+            err[i] = buf[5 + i];
+        }
+    }
+
+    void neighbors(int sid, int level, uint32_t* counts_out, uint32_t* sums_out, size_t n) {
+        int alive = 0;
+        for (auto& x : scenes) alive += x.alive;
+        require(alive == 1 && scenes[sid].alive, "neighbour inspection needs a batch of one scene");
+        tss::DevCounts c = read_counts();
+        const int nl = level ? c.n_h : c.n_l;
+        require(n == (size_t)nl, "n must equal the level's list length (" + std::to_string(nl) + ")");
+        if (nb_cap < 5 * n + 5) {
+            if (nb_counts) (void)hipFree(nb_counts);
+            if (nb_sums) (void)hipFree(nb_sums);
+            nb_counts = nb_sums = nullptr;
+            HIP_CHECK(hipMalloc(&nb_counts, 4 * (5 * n + 5)));
+            HIP_CHECK(hipMalloc(&nb_sums, 4 * (5 * n + 5)));
+            nb_cap = 5 * n + 5;
+        }
+        tss::launch_neighbors_n(d, sh, sid, level, nl, nb_counts, nb_sums, stream);
+        if (n) {
+            HIP_CHECK(hipMemcpyAsync(counts_out, nb_counts, 20 * n, hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(sums_out, nb_sums, 20 * n, hipMemcpyDeviceToHost, stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(stream));
+        HIP_CHECK(hipGetLastError());
+    }
+
+    void* device_ptr(int sid, int field) {
+        FieldView v = view(sid, field);
+        require(v.kind == 0, "field has no plain device view");
+        return v.dev;
+    }
+};
+
+struct tss_solver {
+    tss_batch b;
+    int sid = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Scene generator
+// ---------------------------------------------------------------------------
+namespace {
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+float unit24(uint64_t r) { return (float)(r >> 40) * (1.0f / 16777216.0f); }
+}  // namespace
+
+extern "C" {
+
+uint64_t tss_splitmix64_next(uint64_t* state) {
+    *state += kGolden;
+    return mix64(*state);
+}
+
+int tss_canvas_count(int nxd, int nyd, int layers) {
+    if (nxd < 0 || nyd < 0 || layers < 0) return -1;
+    return (nxd + 2 * layers) * (nyd + 2 * layers) - nxd * nyd;
+}
+
+int tss_scene_dam_break(uint64_t seed, uint64_t scene, int nx, int ny, float dx, float jitter,
+                        int nxd, int nyd, int layers, float* low_xy, float* bnd_xy) {
+    return guarded([&] {
+        require(nx >= 0 && ny >= 0 && dx > 0 && jitter >= 0 && nxd >= 0 && nyd >= 0 && layers >= 0,
+                "invalid scene generator arguments");
+        const float amp = jitter * dx;
+        for (int j = 0; j < ny; ++j) {
+            for (int i = 0; i < nx; ++i) {
+                const uint64_t idx = (uint64_t)j * (uint64_t)nx + (uint64_t)i;
+                uint64_t st = mix64(mix64(seed + kGolden * (scene + 1)) + kGolden * (idx + 1));
+                float ux = unit24(tss_splitmix64_next(&st));
+                float uy = unit24(tss_splitmix64_next(&st));
+                float jx = (2.0f * ux - 1.0f) * amp;
+                float jy = (2.0f * uy - 1.0f) * amp;
+                low_xy[2 * idx] = (0.5f + (float)i) * dx + jx;
+                low_xy[2 * idx + 1] = (0.5f + (float)j) * dx + jy;
+            }
+        }
+        if (bnd_xy) {
+            size_t k = 0;
+            for (int j = -layers; j < nyd + layers; ++j) {
+                for (int i = -layers; i < nxd + layers; ++i) {
+                    if (i < 0 || i >= nxd || j < 0 || j >= nyd) {
+                        bnd_xy[2 * k] = (0.5f + (float)i) * dx;
+                        bnd_xy[2 * k + 1] = (0.5f + (float)j) * dx;
+                        ++k;
+                    }
+                }
+            }
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Configuration (WCSPHConfig, R/WCSPHSolver.cu:2287-2332)
+// ---------------------------------------------------------------------------
+int tss_config_init(tss_config* cfg, float xs, float ys, float xe, float ye, float h, float H,
+                    float rho0, float B, float c, float alpha, float sigma, float dt) {
+    return guarded([&] {
+        require(cfg != nullptr, "null config");
+        std::memset(cfg, 0, sizeof(*cfg));
+        require(!(xs >= xe || ys >= ye), "Invalid configuration parameters");
+        require(h > 0 && H > 0, "Invalid configuration parameters: radii must be > 0");
+        cfg->domain_origin[0] = xs;
+        cfg->domain_origin[1] = ys;
+        cfg->domain_end[0] = xe;
+        cfg->domain_end[1] = ye;
+        cfg->domain_dims[0] = (int)std::ceil((xe - xs) / h);
+        cfg->domain_dims[1] = (int)std::ceil((ye - ys) / h);
+        cfg->domain_dims_high[0] = (int)std::ceil((xe - xs) / H);
+        cfg->domain_dims_high[1] = (int)std::ceil((ye - ys) / H);
+        cfg->effective_radius = h;
+        cfg->effective_radius_high = H;
+        cfg->rest_density = rho0;
+        cfg->tait_coefficient = B;
+        cfg->speed_sound = c;
+        cfg->alpha = alpha;
+        cfg->tension_coefficient = sigma;
+        cfg->time_step = dt;
+        cfg->gravity = 9.81f;
+        cfg->mode = TSS_MODE_PLAIN;
+        cfg->split_x = 0.5f;
+        cfg->surface_ratio = 0.9f;
+        cfg->surface_ratio_high = 0.9f * (float)high_density_ratio((double)H / (double)h);
+        cfg->band_radius = h;
+        cfg->merge_radius = h;
+        cfg->promote_min_age = 5;
+        cfg->promote_max_age = 0;  // promotion off (opt-in)
+        cfg->promote_fluc = 0.05f;
+        cfg->cap_high = 0;
+        cfg->dt_mode = TSS_DT_FIXED;
+        cfg->cfl = 0.4f;
+        cfg->dt_max = dt;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Single-scene solver
+// ---------------------------------------------------------------------------
+int tss_create(const tss_config* cfg, const float* low_xy, int n_low, int cap_low,
+               const float* bnd_xy, int n_bnd, tss_solver** out) {
+    return guarded([&] {
+        require(cfg && out && (low_xy || n_low == 0) && (bnd_xy || n_bnd == 0), "null argument");
+        *out = nullptr;
+        tss_batch::validate(*cfg, n_low, cap_low, n_bnd);
+        auto s = std::make_unique<tss_solver>();
+        tss_batch_desc ds{};
+        ds.max_scenes = 1;
+        ds.cap_low = cap_low;
+        ds.cap_bnd = n_bnd;
+        ds.cells_low = (long long)cfg->domain_dims[0] * cfg->domain_dims[1];
+        ds.cells_high = (long long)cfg->domain_dims_high[0] * cfg->domain_dims_high[1];
+        s->b.create(ds);
+        s->sid = s->b.add_scene(*cfg, low_xy, n_low, cap_low, bnd_xy, n_bnd);
+        *out = s.release();
+    });
+}
+
+int tss_step(tss_solver* s, int n) {
+    return guarded([&] {
+        require(s && n >= 0, "invalid argument");
+        s->b.step_eager(n);
+    });
+}
+
+int tss_step_graph(tss_solver* s, int n) {
+    return guarded([&] {
+        require(s && n >= 0, "invalid argument");
+        s->b.step_graph(n);
+    });
+}
+
+int tss_counts(tss_solver* s, tss_counts_t* o) {
+    return guarded([&] {
+        require(s && o, "null argument");
+        s->b.counts(s->sid, o);
+    });
+}
+
+int tss_read(tss_solver* s, int field, void* dst, size_t bytes) {
+    return guarded([&] {
+        require(s && (dst || bytes == 0), "null argument");
+        s->b.read_field(s->sid, field, dst, bytes);
+    });
+}
+
+int tss_write(tss_solver* s, int field, const void* src, size_t bytes) {
+    return guarded([&] {
+        require(s && (src || bytes == 0), "null argument");
+        s->b.write_field(s->sid, field, src, bytes);
+    });
+}
+
+int tss_set_counts(tss_solver* s, int nl, int nh, int ng) {
+    return guarded([&] {
+        require(s, "null argument");
+        s->b.set_counts(s->sid, nl, nh, ng);
+    });
+}
+
+int tss_device_ptr(tss_solver* s, int field, void** out) {
+    return guarded([&] {
+        require(s && out, "null argument");
+        *out = s->b.device_ptr(s->sid, field);
+    });
+}
+
+int tss_synchronize(tss_solver* s) {
+    return guarded([&] {
+        require(s, "null argument");
+        HIP_CHECK(hipStreamSynchronize(s->b.stream));
+    });
+}
+
+int tss_conservation(tss_solver* s, double out[5], double err[5]) {
+    return guarded([&] {
+        require(s && out && err, "null argument");
+        s->b.conservation(s->sid, out, err);
+    });
+}
+
+int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums, size_t n) {
+    return guarded([&] {
+        require(s && (level == 0 || level == 1) && ((counts && sums) || n == 0), "invalid argument");
+        s->b.neighbors(s->sid, level, counts, sums, n);
+    });
+}
+
+int tss_enable_timing(tss_solver* s, int on) {
+    return guarded([&] {
+        require(s, "null argument");
+        s->b.timing = on != 0;
+    });
+}
+
+int tss_phase_times(tss_solver* s, float out[5]) {
+    return guarded([&] {
+        require(s && out, "null argument");
+        for (int i = 0; i < 5; ++i) out[i] = s->b.phase_ms[i];
+    });
+}
+
+void tss_destroy(tss_solver* s) { delete s; }
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+int tss_batch_memory_bytes(const tss_batch_desc* d, long long* bytes) {
+    return guarded([&] {
+        require(d && bytes, "null argument");
+        *bytes = tss_batch::bytes_for(*d);
+    });
+}
+
+int tss_batch_create(const tss_batch_desc* d, tss_batch** out) {
+    return guarded([&] {
+        require(d && out, "null argument");
+        *out = nullptr;
+        auto b = std::make_unique<tss_batch>();
+        b->create(*d);
+        *out = b.release();
+    });
+}
+
+int tss_batch_add_scene(tss_batch* b, const tss_config* cfg, const float* low_xy, int n_low,
+                        int cap_low, const float* bnd_xy, int n_bnd, int* scene_id) {
+    return guarded([&] {
+        require(b && cfg && scene_id && (low_xy || n_low == 0) && (bnd_xy || n_bnd == 0),
+                "null argument");
+        *scene_id = b->add_scene(*cfg, low_xy, n_low, cap_low, bnd_xy, n_bnd);
+    });
+}
+
+int tss_batch_remove_scene(tss_batch* b, int sid) {
+    return guarded([&] {
+        require(b, "null argument");
+        b->remove_scene(sid);
+    });
+}
+
+int tss_batch_step(tss_batch* b, int n) {
+    return guarded([&] {
+        require(b && n >= 0, "invalid argument");
+        b->step_eager(n);
+    });
+}
+
+int tss_batch_step_graph(tss_batch* b, int n) {
+    return guarded([&] {
+        require(b && n >= 0, "invalid argument");
+        b->step_graph(n);
+    });
+}
+
+int tss_batch_counts(tss_batch* b, int sid, tss_counts_t* o) {
+    return guarded([&] {
+        require(b && o, "null argument");
+        b->counts(sid, o);
+    });
+}
+
+int tss_batch_read(tss_batch* b, int sid, int field, void* dst, size_t bytes) {
+    return guarded([&] {
+        require(b && (dst || bytes == 0), "null argument");
+        b->read_field(sid, field, dst, bytes);
+    });
+}
+
+int tss_batch_conservation(tss_batch* b, int sid, double out[5], double err[5]) {
+    return guarded([&] {
+        require(b && out && err, "null argument");
+        b->conservation(sid, out, err);
+    });
+}
+
+int tss_batch_synchronize(tss_batch* b) {
+    return guarded([&] {
+        require(b, "null argument");
+        HIP_CHECK(hipStreamSynchronize(b->stream));
+    });
+}
+
+int tss_batch_num_scenes(tss_batch* b, int* n) {
+    return guarded([&] {
+        require(b && n, "null argument");
+        int k = 0;
+        for (auto& s : b->scenes) k += s.alive;
+        *n = k;
+    });
+}
+
+int tss_batch_set_tuning(tss_batch* b, int dens_block, int force_block) {
+    return guarded([&] {
+        require(b, "null argument");
+        auto ok = [](int x) { return x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
+        require(ok(dens_block) && ok(force_block), "block sizes must be 64..1024 powers of two");
+        b->dens_block = dens_block;
+        b->force_block = force_block;
+        b->update_shape();
+    });
+}
+
+void tss_batch_destroy(tss_batch* b) { delete b; }
+
+const char* tss_last_error(void) { return g_last_error.c_str(); }
+
+const char* tss_version(void) { return "tss 0.1.0 gfx950"; }
+
+}  // extern "C"

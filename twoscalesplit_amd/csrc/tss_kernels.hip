@@ -1,0 +1,1463 @@
+// tss_kernels.hip -- HIP kernels (gfx950) of the two-scale WCSPH step.
+//
+// Reference: segfault11/TwoScaleSplit, TwoScale2D/WCSPHSolver.cu ("R:" lines).
+// Compiled with -ffp-contract=off: every expression that decides an index or
+// a set (cell coordinates, pair distances, cutoffs, integration, child
+// placement) rounds each operation exactly like the FP32 method definition;
+// the physics sums use explicit fmaf where a fused operation is wanted.
+//
+// Step (one launch per phase for all scenes of a batch):
+//   fill -> hash -> radix sort (P passes x {hist, scan, scatter})
+//   -> ranges+reorder -> density -> force+integrate -> reduce
+//   -> split candidates -> groups -> cap -> compaction (3 kernels)
+//   -> finalize
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "tss_device.h"
+
+namespace tss {
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+#define GRID_STRIDE(k, n)                                             \
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < (n);     \
+         k += gridDim.x * blockDim.x)
+
+constexpr uint8_t kActive = 0, kTransient = 1, kDeleted = 2;
+
+// Grid coordinate along one axis: clamp((int)(((x+off)-o)/cell), 0, dim-1)
+// (R:241-289); the clamp is taken in float so every input is defined.
+__device__ __forceinline__ int axis_cell(float x, float off, float o,
+                                         float cell, int dim) {
+    float t = ((x + off) - o) / cell;
+    if (!(t >= 1.0f)) return 0;
+    if (t >= (float)dim) return dim - 1;
+    return (int)t;
+}
+
+// Squared pair distance, each operation rounded (R:325-333 without sqrt:
+// sqrt(r2) <= R  <=>  r2 <= T(R) for the precomputed threshold T).
+__device__ __forceinline__ float dist2(float2 a, float2 b) {
+    float dx = a.x - b.x;
+    float dy = a.y - b.y;
+    float p = dx * dx;
+    float q = dy * dy;
+    return p + q;
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+struct Box {
+    int x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ Box scan_box(const SceneDev& sc, bool high, float2 x,
+                                        float r) {
+    const float cell = high ? sc.H : sc.h;
+    const int dx = high ? sc.dhx : sc.dlx;
+    const int dy = high ? sc.dhy : sc.dly;
+    Box b;
+    b.x0 = axis_cell(x.x, -r, sc.ox, cell, dx);
+    b.x1 = axis_cell(x.x, r, sc.ox, cell, dx);
+    b.y0 = axis_cell(x.y, -r, sc.oy, cell, dy);
+    b.y1 = axis_cell(x.y, r, sc.oy, cell, dy);
+    return b;
+}
+
+// Contiguous sorted range covering cells x0..x1 of one grid row (cells are
+// row-major, so neighbouring cells of a row are adjacent in sorted order).
+__device__ __forceinline__ void row_run(const int* __restrict__ cs,
+                                        const int* __restrict__ ce, int row,
+                                        int x0, int x1, int& s, int& e) {
+    s = -1;
+    e = -1;
+    for (int cx = x0; cx <= x1; ++cx) {
+        int st = cs[row + cx];
+        if (st >= 0) {
+            if (s < 0) s = st;
+            e = ce[row + cx];
+        }
+    }
+}
+
+// Blended M4 kernel (R:116-239, 358-361): W_bar = (W_h + W_H)/2 and the
+// factor G with grad W_bar = G/r * x_ij. Branch-free: with a=max(2-q,0),
+// b=max(1-q,0), f = a^3 - 4b^3 and f' = a^2 - 4b^2 on all of q >= 0.
+__device__ __forceinline__ void m4_blend(const SceneDev& sc, float r, float& w,
+                                         float& g) {
+    float qh = r * sc.q_h, qH = r * sc.q_H;
+    float ah = fmaxf(2.0f - qh, 0.0f), bh = fmaxf(1.0f - qh, 0.0f);
+    float aH = fmaxf(2.0f - qH, 0.0f), bH = fmaxf(1.0f - qH, 0.0f);
+    float ah2 = ah * ah, bh2 = bh * bh, aH2 = aH * aH, bH2 = bH * bH;
+    float fh = fmaf(-4.0f * bh, bh2, ah2 * ah);
+    float fH = fmaf(-4.0f * bH, bH2, aH2 * aH);
+    float dh = fmaf(-4.0f, bh2, ah2);
+    float dH = fmaf(-4.0f, bH2, aH2);
+    w = fmaf(sc.sig_h2, fh, sc.sig_H2 * fH);
+    g = fmaf(sc.gc_h2, dh, sc.gc_H2 * dH);
+}
+
+// Wall weight Gamma(r)/(mass factor) (R:66-114), qs = 2/ell.
+__device__ __forceinline__ float wall_gamma(const SceneDev& sc, float r2,
+                                            float r, float qs) {
+    float q = r * qs;
+    float c = sc.cwall / r2;
+    float v;
+    if (q < 2.0f / 3.0f) {
+        v = 2.0f / 3.0f;
+    } else if (q < 1.0f) {
+        v = q * fmaf(-1.5f, q, 2.0f);
+    } else if (q < 2.0f) {
+        float a = 2.0f - q;
+        v = 0.5f * a * a;
+    } else {
+        v = 0.0f;
+    }
+    return c * v;
+}
+
+// Tait pressure from a compensated density hi+lo (R:1421-1423 restated as
+// B((1+d)^7 - 1) with d = ((hi - rho0) + lo)/rho0, binomially expanded so
+// small |d| keeps its relative accuracy).
+__device__ __forceinline__ float tait(const SceneDev& sc, float hi, float lo) {
+    float d = ((hi - sc.rho0) + lo) / sc.rho0;
+    float poly = fmaf(d, 1.0f, 7.0f);
+    poly = fmaf(d, poly, 21.0f);
+    poly = fmaf(d, poly, 35.0f);
+    poly = fmaf(d, poly, 35.0f);
+    poly = fmaf(d, poly, 21.0f);
+    poly = fmaf(d, poly, 7.0f);
+    return sc.B * d * poly;
+}
+
+// hi+lo of m1*(s1-c1) + m2*(s2-c2) where (s, c) are Kahan sums.
+__device__ __forceinline__ void mass_sum(float m1, float s1, float c1,
+                                         float m2, float s2, float c2,
+                                         float& hi, float& lo) {
+    float h1 = m1 * s1;
+    float l1 = fmaf(m1, s1, -h1) - m1 * c1;
+    float h2 = m2 * s2;
+    float l2 = fmaf(m2, s2, -h2) - m2 * c2;
+    float s = h1 + h2;
+    float bb = s - h1;
+    float e = (h1 - (s - bb)) + (h2 - bb);
+    hi = s;
+    lo = (l1 + l2) + e;
+}
+
+__device__ __forceinline__ float fast_rsq(float x) {
+    return __builtin_amdgcn_rsqf(x);
+}
+
+__device__ __forceinline__ float fast_rcp(float x) {
+    return __builtin_amdgcn_rcpf(x);
+}
+
+// |x_ij| from r2 >= 0 without the 0 * inf of r2 * rsq(r2) at r2 = 0 (the
+// self pair of the density sums).
+__device__ __forceinline__ float dist_from2(float r2) {
+    return r2 > 0.0f ? r2 * fast_rsq(r2) : 0.0f;
+}
+
+__device__ __forceinline__ int low_of(const DevState& d, int u) {
+    return u < d.CL ? u : (u - d.CL) >> 2;
+}
+
+// ---------------------------------------------------------------------------
+// Fill kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_int(int* __restrict__ p, int v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Start of a step: clear cell tables and per-scene step accumulators.
+__global__ void k_step_begin(DevState d, long long ncells) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         i < ncells; i += (long long)gridDim.x * blockDim.x) {
+        d.cstart[i] = -1;
+        d.cend[i] = -1;
+    }
+    GRID_STRIDE(s, d.S) {
+        SceneCount& c = d.scnt[s];
+        c.n_split = 0;
+        c.n_promote = 0;
+        c.n_merge = 0;
+        c.n_dissolve = 0;
+        c.cand = 0;
+        c.seg_l0 = 0;
+        c.seg_l1 = 0;
+        c.seg_h0 = 0;
+        c.seg_h1 = 0;
+        c.kl0 = c.kl1 = c.kh0 = c.kh1 = c.sp0 = c.sp1 = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hash (R:1246-1298): key = unified cell index of the particle.
+// ---------------------------------------------------------------------------
+__global__ void k_hash(DevState d, const int* __restrict__ list_l,
+                       const int* __restrict__ list_h,
+                       uint32_t* __restrict__ keys,
+                       uint32_t* __restrict__ vals) {
+    const int nl = d.counts->n_l;
+    const int n = nl + d.counts->n_h;
+    GRID_STRIDE(k, n) {
+        int u = k < nl ? list_l[k] : list_h[k - nl];
+        bool high = u >= d.CL;
+        int s = d.scene_of[low_of(d, u)];
+        const SceneDev& sc = d.scenes[s];
+        float2 x = d.pos[u];
+        uint32_t key;
+        if (!high) {
+            int cx = axis_cell(x.x, 0.0f, sc.ox, sc.h, sc.dlx);
+            int cy = axis_cell(x.y, 0.0f, sc.oy, sc.h, sc.dly);
+            key = (uint32_t)(sc.cbase_l + cy * sc.dlx + cx);
+        } else {
+            int cx = axis_cell(x.x, 0.0f, sc.ox, sc.H, sc.dhx);
+            int cy = axis_cell(x.y, 0.0f, sc.oy, sc.H, sc.dhy);
+            key = (uint32_t)(sc.cbase_h + cy * sc.dhx + cx);
+        }
+        keys[k] = key;
+        vals[k] = (uint32_t)u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits (replaces thrust::sort_by_key,
+// R:2828-2834, 2962-2968). Per pass: tile histograms -> per-digit scan over
+// tiles -> stable scatter ranked with wavefront ballots.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(
+    const uint32_t* __restrict__ keys, const DevCounts* __restrict__ cnt,
+    int shift, uint32_t* __restrict__ hist, int max_tiles) {
+    __shared__ uint32_t h[256];
+    const int n = cnt->n_l + cnt->n_h;
+    const int tile = blockIdx.x;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = tile * kSortTile;
+    if (base < n) {
+        for (int i = 0; i < kSortItems; ++i) {
+            int e = base + i * kSortThreads + threadIdx.x;
+            if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    hist[threadIdx.x * max_tiles + tile] = h[threadIdx.x];
+}
+
+// One block per digit: exclusive scan over the tiles of that digit.
+__global__ __launch_bounds__(256) void k_sort_scan(
+    uint32_t* __restrict__ hist, const DevCounts* __restrict__ cnt,
+    int max_tiles, uint32_t* __restrict__ digit_tot) {
+    __shared__ uint32_t part[256];
+    const int n = cnt->n_l + cnt->n_h;
+    const int nt = (n + kSortTile - 1) / kSortTile;
+    uint32_t* row = hist + (size_t)blockIdx.x * max_tiles;
+    const int per = (nt + 255) / 256;
+    const int t0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (int i = 0; i < per; ++i) {
+        int t = t0 + i;
+        if (t < nt) local += row[t];
+    }
+    part[threadIdx.x] = local;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 256 partials
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - local;
+    for (int i = 0; i < per; ++i) {
+        int t = t0 + i;
+        if (t < nt) {
+            uint32_t v = row[t];
+            row[t] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 255) digit_tot[blockIdx.x] = part[255];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+    const DevCounts* __restrict__ cnt, int shift,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_tot,
+    int max_tiles) {
+    constexpr int kWaves = kSortThreads / 64;
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wcnt[kWaves][256];
+    __shared__ uint32_t woff[kWaves][256];
+    const int n = cnt->n_l + cnt->n_h;
+    const int tile = blockIdx.x;
+    const int tbase = tile * kSortTile;
+    if (tbase >= n) return;  // whole block exits together
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    // digit bases: exclusive scan of digit totals + this tile's prefix
+    {
+        uint32_t v = digit_tot[t];
+        base[t] = v;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            uint32_t a = t >= off ? base[t - off] : 0u;
+            __syncthreads();
+            base[t] += a;
+            __syncthreads();
+        }
+        uint32_t excl = base[t] - v;
+        __syncthreads();
+        base[t] = excl + hist[t * max_tiles + tile];
+        running[t] = 0;
+        for (int q = 0; q < kWaves; ++q) wcnt[q][t] = 0;
+    }
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int it = 0; it < kSortItems; ++it) {
+        const int e = tbase + it * kSortThreads + t;
+        const bool valid = e < n;
+        uint32_t key = valid ? kin[e] : 0u;
+        uint32_t val = valid ? vin[e] : 0u;
+        uint32_t dg = (key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bb = __ballot((dg >> b) & 1u);
+            m &= ((dg >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t rk = (uint32_t)__popcll(m & lt);
+        if (valid && rk == 0) wcnt[w][dg] = (uint32_t)__popcll(m);
+        __syncthreads();
+        {
+            uint32_t run = running[t];
+            for (int q = 0; q < kWaves; ++q) {
+                woff[q][t] = run;
+                run += wcnt[q][t];
+                wcnt[q][t] = 0;
+            }
+            running[t] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t dst = base[dg] + woff[w][dg] + rk;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell ranges (R:1300-1340) + reorder into cell-sorted SoA + scene segments.
+// ---------------------------------------------------------------------------
+__global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
+                          const uint32_t* __restrict__ vals) {
+    const int nl = d.counts->n_l;
+    const int n = nl + d.counts->n_h;
+    GRID_STRIDE(k, n) {
+        const uint32_t key = keys[k];
+        const int u = (int)vals[k];
+        if (k == 0 || keys[k - 1] != key) d.cstart[key] = k;
+        if (k == n - 1 || keys[k + 1] != key) d.cend[key] = k + 1;
+        d.spos[k] = d.pos[u];
+        d.svel[k] = d.vel[u];
+        d.sstate[k] = d.state[u];
+        d.sidx[u] = k;
+        const int s = d.scene_of[low_of(d, u)];
+        d.sscene[k] = s;
+        // scene segments per level: low part [0, nl), high part [nl, n)
+        const bool high = k >= nl;
+        const int first = high ? nl : 0;
+        const int last = high ? n - 1 : nl - 1;
+        bool starts = (k == first);
+        if (!starts) {
+            int up = (int)vals[k - 1];
+            starts = d.scene_of[low_of(d, up)] != s;
+        }
+        bool ends = (k == last);
+        if (!ends) {
+            int un = (int)vals[k + 1];
+            ends = d.scene_of[low_of(d, un)] != s;
+        }
+        if (starts) {
+            if (high) d.scnt[s].seg_h0 = k; else d.scnt[s].seg_l0 = k;
+        }
+        if (ends) {
+            if (high) d.scnt[s].seg_h1 = k + 1; else d.scnt[s].seg_l1 = k + 1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Density + Tait pressure (R:335-474, 1342-1516; high<-low reads the low
+// arrays into its own sum: the method's ruling on R:1494-1504).
+// ---------------------------------------------------------------------------
+__global__ void k_density(DevState d, const uint32_t* __restrict__ vals) {
+    const int nl = d.counts->n_l;
+    const int n = nl + d.counts->n_h;
+    const int* __restrict__ cs = d.cstart;
+    const int* __restrict__ ce = d.cend;
+    const float2* __restrict__ spos = d.spos;
+    const uint8_t* __restrict__ sst = d.sstate;
+    GRID_STRIDE(k, n) {
+        const int s = d.sscene[k];
+        const SceneDev& sc = d.scenes[s];
+        const float2 xi = spos[k];
+        const uint8_t si = sst[k];
+        const bool high = k >= nl;
+        float s1 = 0.f, c1 = 0.f, s2 = 0.f, c2 = 0.f;
+        if (!high) {
+            // low <- low, r <= h, every state (R:335-363)
+            Box b = scan_box(sc, false, xi, sc.h);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                row_run(cs, ce, sc.cbase_l + cy * sc.dlx, b.x0, b.x1, st, en);
+                for (int j = st; j < en; ++j) {
+                    float r2 = dist2(xi, spos[j]);
+                    if (r2 <= sc.t_le_h) {
+                        float r = dist_from2(r2);
+                        float w, g;
+                        m4_blend(sc, r, w, g);
+                        kahan_add(s1, c1, w);
+                    }
+                }
+            }
+            // low <- high, r <= h, lambda_LH (R:365-400)
+            if (sc.dhx > 0) {
+                Box bh = scan_box(sc, true, xi, sc.h);
+                for (int cy = bh.y0; cy <= bh.y1; ++cy) {
+                    int st, en;
+                    row_run(cs, ce, sc.cbase_h + cy * sc.dhx, bh.x0, bh.x1, st,
+                            en);
+                    for (int j = st; j < en; ++j) {
+                        float r2 = dist2(xi, spos[j]);
+                        if (r2 <= sc.t_le_h && !(sst[j] & 1u)) {
+                            float r = dist_from2(r2);
+                            float w, g;
+                            m4_blend(sc, r, w, g);
+                            kahan_add(s2, c2, w);
+                        }
+                    }
+                }
+            }
+        } else {
+            // high <- high, r <= H, lambda_HH (R:402-437)
+            Box b = scan_box(sc, true, xi, sc.H);
+            const bool i_prom = !(si & 1u);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                row_run(cs, ce, sc.cbase_h + cy * sc.dhx, b.x0, b.x1, st, en);
+                for (int j = st; j < en; ++j) {
+                    float r2 = dist2(xi, spos[j]);
+                    if (r2 <= sc.t_le_H && !(i_prom && (sst[j] & 1u))) {
+                        float r = dist_from2(r2);
+                        float w, g;
+                        m4_blend(sc, r, w, g);
+                        kahan_add(s1, c1, w);
+                    }
+                }
+            }
+            // high <- low, r <= H, lambda_HL (R:439-474, low arrays)
+            Box bl = scan_box(sc, false, xi, sc.h);
+            for (int cy = bl.y0; cy <= bl.y1; ++cy) {
+                int st, en;
+                row_run(cs, ce, sc.cbase_l + cy * sc.dlx, bl.x0, bl.x1, st, en);
+                for (int j = st; j < en; ++j) {
+                    float r2 = dist2(xi, spos[j]);
+                    if (r2 <= sc.t_le_H && !((si & 1u) && (sst[j] & 1u))) {
+                        float r = dist_from2(r2);
+                        float w, g;
+                        m4_blend(sc, r, w, g);
+                        kahan_add(s2, c2, w);
+                    }
+                }
+            }
+        }
+        float hi, lo;
+        if (!high)
+            mass_sum(sc.m_l, s1, c1, sc.m_h, s2, c2, hi, lo);
+        else
+            mass_sum(sc.m_h, s1, c1, sc.m_l, s2, c2, hi, lo);
+        const float rho = hi + lo;
+        const float pr = tait(sc, hi, lo);
+        const int u = (int)vals[k];
+        d.srho[k] = rho;
+        d.spr[k] = pr / (rho * rho);
+        d.rho[u] = rho;
+        d.p[u] = pr;
+        // read-back quantity of setQuantities (R:1030-1051)
+        d.vis[u] = fminf(fabsf(rho - sc.rho0), 400.0f) / 400.0f;
+        // free-surface mark: density deficit, listed low or promoted high,
+        // and no wall particle within h (walls carry no density)
+        uint8_t surf = high ? (!(si & 1u) && rho < sc.surf_t_h)
+                            : (rho < sc.surf_t);
+        if (surf) {
+            Box b = scan_box(sc, false, xi, sc.h);
+            for (int cy = b.y0; cy <= b.y1 && surf; ++cy) {
+                int bs, be;
+                row_run(d.bstart, d.bend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1,
+                        bs, be);
+                for (int j = bs; j < be; ++j) {
+                    if (dist2(xi, d.bpos[j]) <= sc.t_le_h) {
+                        surf = 0;
+                        break;
+                    }
+                }
+            }
+        }
+        d.ssurf[k] = surf;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forces + symplectic Euler (R:476-885, 1604-1646, 1648-1865, 1867-2033).
+// All terms read the cell-sorted pre-step copies; results go to the by-id
+// arrays, so there is no read/write race between particles.
+// ---------------------------------------------------------------------------
+struct Acc2 {
+    float s_x, c_x, s_y, c_y;  // compensated totals
+};
+
+__device__ __forceinline__ void acc_flush(Acc2& a, float& rx, float& ry) {
+    kahan_add(a.s_x, a.c_x, rx);
+    kahan_add(a.s_y, a.c_y, ry);
+    rx = 0.f;
+    ry = 0.f;
+}
+
+__device__ __forceinline__ float acc_get_x(const Acc2& a) { return a.s_x - a.c_x; }
+__device__ __forceinline__ float acc_get_y(const Acc2& a) { return a.s_y - a.c_y; }
+
+// One fluid pair (R:502-579 and its level variants): pressure + Monaghan
+// viscosity times grad W_bar, and the tension kernel term.
+__device__ __forceinline__ void fluid_pair(const SceneDev& sc, float2 xi,
+                                           float2 vi, float rhoi, float pri,
+                                           float2 xj, float2 vj, float rhoj,
+                                           float prj, float r2, float visc,
+                                           float eps, float& ax, float& ay,
+                                           float& tx, float& ty) {
+    float dx = xi.x - xj.x;
+    float dy = xi.y - xj.y;
+    float rinv = fast_rsq(r2);
+    float r = r2 * rinv;
+    float w, g;
+    m4_blend(sc, r, w, g);
+    float coeff = pri + prj;
+    float dvp = fmaf(vi.x - vj.x, dx, (vi.y - vj.y) * dy);
+    if (dvp < 0.0f) {
+        coeff -= dvp * visc * fast_rcp((r2 + eps) * (rhoi + rhoj));
+    }
+    float t = coeff * g * rinv;
+    ax = fmaf(t, dx, ax);
+    ay = fmaf(t, dy, ay);
+    tx = fmaf(w, dx, tx);
+    ty = fmaf(w, dy, ty);
+}
+
+__global__ void k_force(DevState d, const uint32_t* __restrict__ vals) {
+    const int nl = d.counts->n_l;
+    const int n = nl + d.counts->n_h;
+    const int* __restrict__ cs = d.cstart;
+    const int* __restrict__ ce = d.cend;
+    const float2* __restrict__ spos = d.spos;
+    const float2* __restrict__ svel = d.svel;
+    const float* __restrict__ srho = d.srho;
+    const float* __restrict__ spr = d.spr;
+    const uint8_t* __restrict__ sst = d.sstate;
+    GRID_STRIDE(k, n) {
+        const int s = d.sscene[k];
+        const SceneDev& sc = d.scenes[s];
+        const float2 xi = spos[k];
+        const float2 vi = svel[k];
+        const float rhoi = srho[k];
+        const float pri = spr[k];
+        const uint8_t si = sst[k];
+        const bool high = k >= nl;
+        Acc2 a1 = {0.f, 0.f, 0.f, 0.f};  // same-level pressure/viscosity
+        Acc2 a2 = {0.f, 0.f, 0.f, 0.f};  // cross-level
+        Acc2 at = {0.f, 0.f, 0.f, 0.f};  // tension
+        float bx = 0.f, by = 0.f;        // wall
+        float rx, ry, tx, ty;
+        if (!high) {
+            Box b = scan_box(sc, false, xi, sc.h);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                const int row = sc.cbase_l + cy * sc.dlx;
+                int st, en;
+                row_run(cs, ce, row, b.x0, b.x1, st, en);
+                rx = ry = tx = ty = 0.f;
+                for (int j = st; j < en; ++j) {
+                    float2 xj = spos[j];
+                    float r2 = dist2(xi, xj);
+                    if (r2 != 0.0f && r2 <= sc.t_lt_h) {
+                        // deleted (state 2) particles are never listed, so
+                        // the reference's deleted[] weight is 1 here
+                        fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j], srho[j],
+                                   spr[j], r2, sc.visc_h, sc.eps_h, rx, ry, tx,
+                                   ty);
+                    }
+                }
+                acc_flush(a1, rx, ry);
+                acc_flush(at, tx, ty);
+                // wall particles of the same low cells (R:565-579)
+                int bs, be;
+                row_run(d.bstart, d.bend, row, b.x0, b.x1, bs, be);
+                for (int j = bs; j < be; ++j) {
+                    float2 xb = d.bpos[j];
+                    float dx = xi.x - xb.x, dy = xi.y - xb.y;
+                    float r2 = dist2(xi, xb);
+                    float r = dist_from2(r2);
+                    float gm = sc.cb_l * wall_gamma(sc, r2, r, sc.q_h);
+                    bx = fmaf(gm, dx, bx);
+                    by = fmaf(gm, dy, by);
+                }
+            }
+            if (d.scnt[s].seg_h1 > d.scnt[s].seg_h0) {
+                Box bh = scan_box(sc, true, xi, sc.h);
+                for (int cy = bh.y0; cy <= bh.y1; ++cy) {
+                    int st, en;
+                    row_run(cs, ce, sc.cbase_h + cy * sc.dhx, bh.x0, bh.x1, st,
+                            en);
+                    rx = ry = tx = ty = 0.f;
+                    for (int j = st; j < en; ++j) {
+                        float2 xj = spos[j];
+                        float r2 = dist2(xi, xj);
+                        if (r2 != 0.0f && r2 <= sc.t_lt_h && !(sst[j] & 1u)) {
+                            fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j],
+                                       srho[j], spr[j], r2, sc.visc_x,
+                                       sc.eps_x, rx, ry, tx, ty);
+                        }
+                    }
+                    acc_flush(a2, rx, ry);
+                    acc_flush(at, tx, ty);
+                }
+            }
+        } else {
+            const bool i_prom = !(si & 1u);
+            Box b = scan_box(sc, true, xi, sc.H);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                row_run(cs, ce, sc.cbase_h + cy * sc.dhx, b.x0, b.x1, st, en);
+                rx = ry = tx = ty = 0.f;
+                for (int j = st; j < en; ++j) {
+                    float2 xj = spos[j];
+                    float r2 = dist2(xi, xj);
+                    if (r2 != 0.0f && r2 <= sc.t_lt_H &&
+                        !(i_prom && (sst[j] & 1u))) {
+                        fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j], srho[j],
+                                   spr[j], r2, sc.visc_H, sc.eps_H, rx, ry, tx,
+                                   ty);
+                    }
+                }
+                acc_flush(a1, rx, ry);
+                acc_flush(at, tx, ty);
+            }
+            Box bl = scan_box(sc, false, xi, sc.h);
+            for (int cy = bl.y0; cy <= bl.y1; ++cy) {
+                const int row = sc.cbase_l + cy * sc.dlx;
+                int st, en;
+                row_run(cs, ce, row, bl.x0, bl.x1, st, en);
+                rx = ry = tx = ty = 0.f;
+                for (int j = st; j < en; ++j) {
+                    float2 xj = spos[j];
+                    float r2 = dist2(xi, xj);
+                    const uint8_t sj = sst[j];
+                    // lambda_HL * deleted[s_j] (R:808-809)
+                    if (r2 != 0.0f && r2 <= sc.t_lt_h &&
+                        !((si & 1u) && (sj & 1u)) && sj != kDeleted) {
+                        fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j], srho[j],
+                                   spr[j], r2, sc.visc_x, sc.eps_x, rx, ry, tx,
+                                   ty);
+                    }
+                }
+                acc_flush(a2, rx, ry);
+                acc_flush(at, tx, ty);
+                int bs, be;
+                row_run(d.bstart, d.bend, row, bl.x0, bl.x1, bs, be);
+                for (int j = bs; j < be; ++j) {
+                    float2 xb = d.bpos[j];
+                    float dx = xi.x - xb.x, dy = xi.y - xb.y;
+                    float r2 = dist2(xi, xb);
+                    float r = dist_from2(r2);
+                    float gm = sc.cb_h * wall_gamma(sc, r2, r, sc.q_H);
+                    bx = fmaf(gm, dx, bx);
+                    by = fmaf(gm, dy, by);
+                }
+            }
+        }
+        // R:1790-1805 (low) / R:1997-2012 (high)
+        const float m_same = high ? sc.m_h : sc.m_l;
+        const float m_cross = high ? sc.m_l : sc.m_h;
+        float ax = -m_same * acc_get_x(a1) - m_cross * acc_get_x(a2);
+        float ay = -m_same * acc_get_y(a1) - m_cross * acc_get_y(a2);
+        ax = fmaf(-sc.sigma, acc_get_x(at), ax) + bx;
+        ay = fmaf(-sc.sigma, acc_get_y(at), ay) + by - sc.g;
+        const int u = (int)vals[k];
+        const float dt = sc.dt;
+        float vx = vi.x + dt * ax;
+        float vy = vi.y + dt * ay;
+        float px = xi.x + dt * vx;
+        float py = xi.y + dt * vy;
+        d.acc[u] = make_float2(ax, ay);
+        d.vel[u] = make_float2(vx, vy);
+        d.pos[u] = make_float2(px, py);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions per scene: conservation sums over real particles (listed
+// low, promoted high) and the max |v|^2 for the CFL time step. Fixed chunk
+// and tree shape -> bitwise reproducible, independent of other scenes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_reduce(DevState d,
+                                                const uint32_t* __restrict__ vals) {
+    __shared__ double sh[kRedFields][256];
+    const int s = blockIdx.y;
+    const int chunk = blockIdx.x;
+    const SceneDev& sc = d.scenes[s];
+    const SceneCount& c = d.scnt[s];
+    const int nl_s = c.seg_l1 - c.seg_l0;
+    const int m = nl_s + (c.seg_h1 - c.seg_h0);
+    double acc[kRedFields] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (sc.alive && m > 0) {
+        const long long i0 = (long long)m * chunk / kRedChunks;
+        const long long i1 = (long long)m * (chunk + 1) / kRedChunks;
+        const double g = (double)sc.g;
+        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            int k = i < nl_s ? c.seg_l0 + (int)i : c.seg_h0 + (int)(i - nl_s);
+            int u = (int)vals[k];
+            bool high = u >= d.CL;
+            float2 v = d.vel[u];
+            double vx = v.x, vy = v.y;
+            double v2 = vx * vx + vy * vy;
+            acc[5] = fmax(acc[5], v2);
+            if (!high || d.sstate[k] == 0) {
+                double mm = high ? (double)sc.m_h : (double)sc.m_l;
+                double y = (double)d.pos[u].y;
+                double ke = 0.5 * (mm * v2);
+                double pe = (mm * g) * y;
+                acc[0] += mm;
+                acc[1] += mm * vx;
+                acc[2] += mm * vy;
+                acc[3] += ke;
+                acc[4] += pe;
+                acc[6] += fabs(mm * vx);
+                acc[7] += fabs(mm * vy);
+                acc[8] += fabs(pe);
+            }
+        }
+    }
+    for (int f = 0; f < kRedFields; ++f) sh[f][threadIdx.x] = acc[f];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) {
+            for (int f = 0; f < kRedFields; ++f) {
+                if (f == 5)
+                    sh[f][threadIdx.x] = fmax(sh[f][threadIdx.x],
+                                              sh[f][threadIdx.x + off]);
+                else
+                    sh[f][threadIdx.x] += sh[f][threadIdx.x + off];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* out = d.red + ((size_t)s * kRedChunks + chunk) * kRedFields;
+        for (int f = 0; f < kRedFields; ++f) out[f] = sh[f][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Adaptivity.
+// ---------------------------------------------------------------------------
+// Any surface particle within sqrt-distance r of x (r2 <= t)? Scans the
+// low box and the high box of radius r on the cell-sorted pre-step state.
+__device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
+                        float r, float t) {
+    Box b = scan_box(sc, false, x, r);
+    for (int cy = b.y0; cy <= b.y1; ++cy) {
+        int st, en;
+        row_run(d.cstart, d.cend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1, st, en);
+        for (int j = st; j < en; ++j)
+            if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
+    }
+    Box bh = scan_box(sc, true, x, r);
+    for (int cy = bh.y0; cy <= bh.y1; ++cy) {
+        int st, en;
+        row_run(d.cstart, d.cend, sc.cbase_h + cy * sc.dhx, bh.x0, bh.x1, st,
+                en);
+        for (int j = st; j < en; ++j)
+            if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
+    }
+    return false;
+}
+
+// Split candidates over the sorted low particles (R:1834-1836 rule or the
+// surface band rule).
+__global__ void k_split_cand(DevState d, const uint32_t* __restrict__ vals) {
+    const int nl = d.counts->n_l;
+    GRID_STRIDE(k, nl) {
+        const int s = d.sscene[k];
+        const SceneDev& sc = d.scenes[s];
+        uint8_t cand = 0;
+        if (sc.mode != 0 && d.sstate[k] == kActive) {
+            if (sc.mode == 1) {
+                cand = d.pos[vals[k]].x > sc.split_x;  // predicted x' (R:1835)
+            } else {
+                cand = in_band(d, sc, d.spos[k], sc.r_band, sc.t_band);
+            }
+        }
+        d.cand[k] = cand;
+        // per-scene candidate count; one atomic per wave when the wave's
+        // active lanes share a scene (the usual case)
+        const int s0 = __builtin_amdgcn_readfirstlane(s);
+        if (__ballot(s != s0) == 0ull) {
+            const uint64_t cm = __ballot(cand != 0);
+            const uint64_t act = __ballot(true);
+            const int lane = threadIdx.x & 63;
+            if (cm && lane == __ffsll((long long)act) - 1)
+                atomicAdd(&d.scnt[s0].cand, (int)__popcll(cm));
+        } else if (cand) {
+            atomicAdd(&d.scnt[s].cand, 1);
+        }
+    }
+}
+
+// R's adjustPosition (R:1007-1022): r and r/dist in double (the reference's
+// min(float, double)); a coincident child is left in place.
+__device__ __forceinline__ float2 clamp_child(float2 ph, float2 pl, float m_l,
+                                              float rho) {
+    float dx = ph.x - pl.x;
+    float dy = ph.y - pl.y;
+    float a = dx * dx;
+    float b = dy * dy;
+    float dist = sqrtf(a + b);
+    if (!(dist > 0.0f)) return ph;
+    double rmax = 0.75 * sqrt((double)m_l / ((double)rho * 3.14159265358979323846));
+    double r = (double)dist < rmax ? (double)dist : rmax;
+    double t = r / (double)dist;
+    return make_float2((float)((double)pl.x + t * (double)dx),
+                       (float)((double)pl.y + t * (double)dy));
+}
+
+// Existing groups: age, promotion (dormant test of R:1053-1095, enabled by
+// the method), child clamp (R:1097-1172), merge decision and application.
+__global__ void k_groups(DevState d, const int* __restrict__ list_g) {
+    const int ng = d.counts->n_g;
+    GRID_STRIDE(q, ng) {
+        const int p = list_g[q];
+        const int s = d.scene_of[p];
+        const SceneDev& sc = d.scenes[s];
+        const int c0 = d.CL + 4 * p;
+        const uint8_t st = d.state[p];
+        // merged: 1 = promoted group merged back (parent re-listed),
+        //         2 = transient group dissolved (parent already listed)
+        uint8_t merged = 0;
+        if (st == kTransient) {
+            int age = d.age[p] + 1;
+            d.age[p] = age;
+            bool promote = false;
+            if (sc.mode == 2 && sc.pmax > 0) {
+                float fluc = 0.0f;
+                for (int k = 0; k < 4; ++k) fluc += fabsf(sc.rho0 - d.rho[c0 + k]);
+                fluc = fluc / (4.0f * sc.rho0);
+                promote = (age >= sc.pmin && fluc <= sc.pfluc) || age >= sc.pmax;
+            }
+            if (promote) {
+                d.state[p] = kDeleted;
+                for (int k = 0; k < 4; ++k) d.state[c0 + k] = 0;
+                atomicAdd(&d.scnt[s].n_promote, 1);
+            } else {
+                bool keep = true;
+                if (sc.mode == 2) {
+                    keep = false;
+                    for (int k = 0; k < 4 && !keep; ++k) {
+                        float2 x = d.spos[d.sidx[c0 + k]];
+                        keep = in_band(d, sc, x, sc.r_merge, sc.t_merge);
+                    }
+                }
+                if (!keep) {
+                    // dissolve: children leave the lists, the parent stays
+                    merged = 2;
+                    d.state[p] = kActive;
+                    d.age[p] = 0;
+                    atomicAdd(&d.scnt[s].n_dissolve, 1);
+                } else {
+                    const float2 pl = d.pos[p];
+                    const float rho = d.rho[p];
+                    for (int k = 0; k < 4; ++k)
+                        d.pos[c0 + k] = clamp_child(d.pos[c0 + k], pl, sc.m_l, rho);
+                }
+            }
+        } else if (st == kDeleted && sc.mode == 2) {
+            bool keep = false;
+            for (int k = 0; k < 4 && !keep; ++k) {
+                float2 x = d.spos[d.sidx[c0 + k]];
+                keep = in_band(d, sc, x, sc.r_merge, sc.t_merge);
+            }
+            if (!keep) {
+                merged = 1;
+                float2 c[4], v[4];
+                for (int k = 0; k < 4; ++k) {
+                    c[k] = d.pos[c0 + k];
+                    v[k] = d.vel[c0 + k];
+                }
+                d.pos[p] = make_float2(((c[0].x + c[1].x) + (c[2].x + c[3].x)) * 0.25f,
+                                       ((c[0].y + c[1].y) + (c[2].y + c[3].y)) * 0.25f);
+                d.vel[p] = make_float2(((v[0].x + v[1].x) + (v[2].x + v[3].x)) * 0.25f,
+                                       ((v[0].y + v[1].y) + (v[2].y + v[3].y)) * 0.25f);
+                d.state[p] = kActive;
+                d.age[p] = 0;
+                for (int k = 0; k < 4; ++k) d.state[c0 + k] = 1;
+                atomicAdd(&d.scnt[s].n_merge, 1);
+            }
+        }
+        d.merged[p] = merged;
+    }
+}
+
+// High-particle cap (spec): where a scene has more split candidates than
+// its cap allows, keep the first ones in sorted order. One block per scene;
+// exits at once when the cap does not bind.
+__global__ __launch_bounds__(256) void k_cap(DevState d) {
+    __shared__ int part[256];
+    const int s = blockIdx.x;
+    const SceneDev& sc = d.scenes[s];
+    const SceneCount& c = d.scnt[s];
+    if (!sc.alive) return;
+    const long long allowed_ll = ((long long)sc.cap_h - 4LL * c.n_g) / 4;
+    const int allowed = allowed_ll < 0 ? 0 : (int)allowed_ll;
+    if (c.cand <= allowed) return;
+    int run = 0;
+    for (int base = c.seg_l0; base < c.seg_l1; base += 256) {
+        int k = base + threadIdx.x;
+        int f = (k < c.seg_l1) ? d.cand[k] : 0;
+        part[threadIdx.x] = f;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += v;
+            __syncthreads();
+        }
+        int excl = run + part[threadIdx.x] - f;
+        if (f && excl >= allowed) d.cand[k] = 0;
+        int tot = part[255];
+        __syncthreads();
+        run += tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compaction of the next lists (stable; replaces the atomicAdd appends of
+// R:1826, 1845, 2030, 1135-1139). Entries E = [low tiles][high tiles]
+// [group tiles]; two counters per entry:
+//   P0 = keep (low: not deleted; high: group not merged; group: not merged)
+//   P1 = low: split admitted; group: merged.
+// ---------------------------------------------------------------------------
+struct Flags {
+    int p0, p1;
+};
+
+__device__ __forceinline__ Flags entry_flags(const DevState& d,
+                                             const uint32_t* __restrict__ vals,
+                                             const int* __restrict__ list_g,
+                                             int seg, int i, int nl, int nh,
+                                             int ng) {
+    Flags f = {0, 0};
+    if (seg == 0) {
+        if (i < nl) {
+            int u = (int)vals[i];
+            f.p0 = d.state[u] != kDeleted;
+            f.p1 = d.cand[i];
+        }
+    } else if (seg == 1) {
+        if (i < nh) {
+            int u = (int)vals[nl + i];
+            f.p0 = !d.merged[(u - d.CL) >> 2];
+        }
+    } else {
+        if (i < ng) {
+            int p = list_g[i];
+            int m = d.merged[p];
+            f.p0 = m == 0;
+            f.p1 = m == 1;  // only merged-back parents are re-listed
+        }
+    }
+    return f;
+}
+
+__device__ __forceinline__ void tile_seg(const StepShape& sh, int tile,
+                                         int& seg, int& t) {
+    if (tile < sh.scan_tiles_l) {
+        seg = 0;
+        t = tile;
+    } else if (tile < sh.scan_tiles_l + sh.scan_tiles_h) {
+        seg = 1;
+        t = tile - sh.scan_tiles_l;
+    } else {
+        seg = 2;
+        t = tile - sh.scan_tiles_l - sh.scan_tiles_h;
+    }
+}
+
+__device__ __forceinline__ int2 block_excl_scan2(int a, int b, int2* sh) {
+    const int t = threadIdx.x;
+    sh[t] = make_int2(a, b);
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        int2 v = t >= off ? sh[t - off] : make_int2(0, 0);
+        __syncthreads();
+        sh[t].x += v.x;
+        sh[t].y += v.y;
+        __syncthreads();
+    }
+    int2 r = make_int2(sh[t].x - a, sh[t].y - b);
+    return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_compact_reduce(
+    DevState d, StepShape sh, const uint32_t* __restrict__ vals,
+    const int* __restrict__ list_g) {
+    __shared__ int2 part[kScanThreads];
+    const int nl = d.counts->n_l, nh = d.counts->n_h, ng = d.counts->n_g;
+    int seg, t;
+    tile_seg(sh, blockIdx.x, seg, t);
+    int a = 0, b = 0;
+    const int i0 = t * kScanTile + threadIdx.x * kScanItems;
+    for (int q = 0; q < kScanItems; ++q) {
+        Flags f = entry_flags(d, vals, list_g, seg, i0 + q, nl, nh, ng);
+        a += f.p0;
+        b += f.p1;
+    }
+    int2 ex = block_excl_scan2(a, b, part);
+    if (threadIdx.x == kScanThreads - 1) {
+        d.tile_sums[2 * blockIdx.x] = ex.x + a;
+        d.tile_sums[2 * blockIdx.x + 1] = ex.y + b;
+    }
+}
+
+// Single block: exclusive scan of the tile sums, segment bases, new counts.
+__global__ __launch_bounds__(1024) void k_compact_top(DevState d, StepShape sh) {
+    __shared__ int2 part[1024];
+    const int ntiles = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
+    const int per = (ntiles + 1023) / 1024;
+    const int t0 = threadIdx.x * per;
+    int a = 0, b = 0;
+    for (int i = 0; i < per; ++i) {
+        int t = t0 + i;
+        if (t < ntiles) {
+            a += d.tile_sums[2 * t];
+            b += d.tile_sums[2 * t + 1];
+        }
+    }
+    part[threadIdx.x] = make_int2(a, b);
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int2 v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off]
+                                              : make_int2(0, 0);
+        __syncthreads();
+        part[threadIdx.x].x += v.x;
+        part[threadIdx.x].y += v.y;
+        __syncthreads();
+    }
+    int ra = part[threadIdx.x].x - a, rb = part[threadIdx.x].y - b;
+    for (int i = 0; i < per; ++i) {
+        int t = t0 + i;
+        if (t < ntiles) {
+            int va = d.tile_sums[2 * t], vb = d.tile_sums[2 * t + 1];
+            d.tile_base[2 * t] = ra;
+            d.tile_base[2 * t + 1] = rb;
+            ra += va;
+            rb += vb;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int th = sh.scan_tiles_l, tg = sh.scan_tiles_l + sh.scan_tiles_h;
+        const int tot0 = part[1023].x, tot1 = part[1023].y;
+        const int b0h = d.tile_base[2 * th], b1h = d.tile_base[2 * th + 1];
+        const int b0g = d.tile_base[2 * tg], b1g = d.tile_base[2 * tg + 1];
+        const int keepL = b0h, nsplit = b1h;
+        const int keepH = b0g - b0h;
+        const int keepG = tot0 - b0g, nmerge = tot1 - b1g;
+        d.seg_base[0] = b0h;    // P0 base of high segment
+        d.seg_base[1] = b0g;    // P0 base of group segment
+        d.seg_base[2] = b1g;    // P1 base of group segment
+        d.seg_base[3] = keepL;
+        d.seg_base[4] = keepH;
+        d.seg_base[5] = keepG;
+        d.seg_base[6] = nsplit;
+        d.seg_base[7] = nmerge;
+        d.counts->n_l_next = keepL + nmerge;
+        d.counts->n_h_next = keepH + 4 * nsplit;
+        d.counts->n_g_next = keepG + nsplit;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
+    DevState d, StepShape sh, const uint32_t* __restrict__ vals,
+    const int* __restrict__ list_g, int* __restrict__ nlist_l,
+    int* __restrict__ nlist_h, int* __restrict__ nlist_g) {
+    __shared__ int2 part[kScanThreads];
+    const int nl = d.counts->n_l, nh = d.counts->n_h, ng = d.counts->n_g;
+    int seg, t;
+    tile_seg(sh, blockIdx.x, seg, t);
+    Flags f[kScanItems];
+    int a = 0, b = 0;
+    const int i0 = t * kScanTile + threadIdx.x * kScanItems;
+    for (int q = 0; q < kScanItems; ++q) {
+        f[q] = entry_flags(d, vals, list_g, seg, i0 + q, nl, nh, ng);
+        a += f[q].p0;
+        b += f[q].p1;
+    }
+    int2 ex = block_excl_scan2(a, b, part);
+    int g0 = d.tile_base[2 * blockIdx.x] + ex.x;
+    int g1 = d.tile_base[2 * blockIdx.x + 1] + ex.y;
+    const int keepL = d.seg_base[3], keepH = d.seg_base[4],
+              keepG = d.seg_base[5];
+    for (int q = 0; q < kScanItems; ++q) {
+        const int i = i0 + q;
+        if (seg == 0 && i < nl) {
+            const int u = (int)vals[i];
+            const int s = d.sscene[i];
+            SceneCount& sct = d.scnt[s];
+            if (i == sct.seg_l0) {
+                sct.kl0 = g0;
+                sct.sp0 = g1;
+            }
+            if (i == sct.seg_l1 - 1) {
+                sct.kl1 = g0 + f[q].p0;
+                sct.sp1 = g1 + f[q].p1;
+            }
+            if (f[q].p0) nlist_l[g0] = u;
+            if (f[q].p1) {
+                // split (R:956-1005, 1834-1863): children at x' +-(rc, rc)
+                const SceneDev& sc = d.scenes[s];
+                nlist_g[keepG + g1] = u;
+                const float2 x = d.pos[u];
+                const float2 v = d.vel[u];
+                const float rho = d.rho[u];
+                const float two_rho = 2.0f * rho;
+                const float rc = (float)(0.5 * sqrt((double)sc.m_l /
+                                                    ((double)two_rho * 3.14159265358979323846)));
+                const int c0 = d.CL + 4 * u;
+                for (int k = 0; k < 4; ++k) {
+                    const int a2 = c0 + k;
+                    d.pos[a2] = make_float2((k & 1) ? x.x - rc : x.x + rc,
+                                            (k & 2) ? x.y - rc : x.y + rc);
+                    d.vel[a2] = v;
+                    d.acc[a2] = make_float2(0.f, 0.f);
+                    d.rho[a2] = 0.f;
+                    d.p[a2] = 0.f;
+                    d.state[a2] = 1;
+                    nlist_h[keepH + 4 * g1 + k] = a2;
+                }
+                d.state[u] = kTransient;
+                d.age[u] = 0;
+            }
+        } else if (seg == 1 && i < nh) {
+            const int k = nl + i;
+            SceneCount& sct = d.scnt[d.sscene[k]];
+            const int gh = g0 - d.seg_base[0];
+            if (k == sct.seg_h0) sct.kh0 = gh;
+            if (k == sct.seg_h1 - 1) sct.kh1 = gh + f[q].p0;
+            if (f[q].p0) nlist_h[gh] = (int)vals[k];
+        } else if (seg == 2 && i < ng) {
+            const int p = list_g[i];
+            if (f[q].p0) nlist_g[g0 - d.seg_base[1]] = p;
+            if (f[q].p1) nlist_l[keepL + (g1 - d.seg_base[2])] = p;
+        }
+        g0 += f[q].p0;
+        g1 += f[q].p1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// End of step: per-scene dt, conservation sums and counters; global counts.
+// ---------------------------------------------------------------------------
+__global__ void k_finalize(DevState d) {
+    GRID_STRIDE(s, d.S) {
+        SceneDev& sc = d.scenes[s];
+        SceneCount& c = d.scnt[s];
+        if (!sc.alive) continue;
+        double tot[kRedFields] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ch = 0; ch < kRedChunks; ++ch) {
+            const double* r = d.red + ((size_t)s * kRedChunks + ch) * kRedFields;
+            for (int f = 0; f < kRedFields; ++f) {
+                if (f == 5) tot[f] = fmax(tot[f], r[f]);
+                else tot[f] += r[f];
+            }
+        }
+        c.n_l_last = c.seg_l1 - c.seg_l0;
+        c.n_h_last = c.seg_h1 - c.seg_h0;
+        // next lists: low = kept + merged; high = kept + 4 per split
+        c.n_split = c.sp1 - c.sp0;
+        c.n_l = (c.kl1 - c.kl0) + c.n_merge;
+        c.n_h = (c.kh1 - c.kh0) + 4 * c.n_split;
+        c.n_g = c.n_g + c.n_split - c.n_merge - c.n_dissolve;
+        c.step += 1;
+        // rounding bound of the sums: per-term rounding (<= 2u) + the
+        // serial per-thread loop + 8-level tree + kRedChunks serial sum
+        const int m = c.n_l_last + c.n_h_last;
+        const double serial = (double)(m / (kRedChunks * 256) + 1);
+        const double kk = serial + 8.0 + (double)kRedChunks + 2.0;
+        const double u = 1.1102230246251565e-16;
+        const double gam = kk * u / (1.0 - kk * u);
+        double* o = d.cons + (size_t)s * 10;
+        for (int f = 0; f < 5; ++f) o[f] = tot[f];
+        o[5] = gam * tot[0];  // mass (positive terms)
+        o[6] = gam * tot[6];  // sum |m vx|
+        o[7] = gam * tot[7];  // sum |m vy|
+        o[8] = gam * tot[3];  // kinetic (positive terms)
+        o[9] = gam * tot[8];  // sum |m g y|
+        if (sc.dt_mode == 1) {
+            double ell = 0.5 * (double)(c.n_h > 0 ? sc.H : sc.h);
+            double dt = (double)sc.cfl * ell / ((double)sc.c + sqrt(tot[5]));
+            if (dt > (double)sc.dt_max) dt = (double)sc.dt_max;
+            sc.dt = (float)dt;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        DevCounts& g = *d.counts;
+        g.n_l_last = g.n_l;
+        g.n_h_last = g.n_h;
+        g.n_l = g.n_l_next;
+        g.n_h = g.n_h_next;
+        g.n_g = g.n_g_next;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-set inspection (tests): same boxes, runs and predicates as the
+// density and force kernels.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t csum(int id) {
+    return (uint32_t)(id + 1) * 2654435761u;
+}
+
+__global__ void k_neighbors(DevState d, const uint32_t* __restrict__ vals,
+                            const int* __restrict__ list, int n, int scene,
+                            uint32_t* __restrict__ counts,
+                            uint32_t* __restrict__ sums) {
+    const SceneDev& sc = d.scenes[scene];
+    GRID_STRIDE(t, n) {
+        const int u = list[t];
+        const int k = d.sidx[u];
+        const bool high = u >= d.CL;
+        const float2 xi = d.spos[k];
+        uint32_t c[5] = {0, 0, 0, 0, 0}, su[5] = {0, 0, 0, 0, 0};
+        auto local_id = [&](int v) {
+            // scene-local id of a unified id
+            return v < d.CL ? v - sc.lbase : (v - d.CL) - 4 * sc.lbase;
+        };
+        // same level
+        {
+            const float rs = high ? sc.H : sc.h;
+            Box b = scan_box(sc, high, xi, rs);
+            const float tle = high ? sc.t_le_H : sc.t_le_h;
+            const float tlt = high ? sc.t_lt_H : sc.t_lt_h;
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                int row = high ? sc.cbase_h + cy * sc.dhx : sc.cbase_l + cy * sc.dlx;
+                row_run(d.cstart, d.cend, row, b.x0, b.x1, st, en);
+                for (int j = st; j < en; ++j) {
+                    float r2 = dist2(xi, d.spos[j]);
+                    int id = local_id((int)vals[j]);
+                    if (r2 <= tle) { c[0]++; su[0] += csum(id); }
+                    if (r2 != 0.0f && r2 <= tlt) { c[2]++; su[2] += csum(id); }
+                }
+            }
+        }
+        // cross level (box +-h on the other grid)
+        {
+            const bool oh = !high;
+            Box b = scan_box(sc, oh, xi, sc.h);
+            const float tle = high ? sc.t_le_H : sc.t_le_h;
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                int row = oh ? sc.cbase_h + cy * sc.dhx : sc.cbase_l + cy * sc.dlx;
+                row_run(d.cstart, d.cend, row, b.x0, b.x1, st, en);
+                for (int j = st; j < en; ++j) {
+                    float r2 = dist2(xi, d.spos[j]);
+                    int id = local_id((int)vals[j]);
+                    if (r2 <= tle) { c[1]++; su[1] += csum(id); }
+                    if (r2 != 0.0f && r2 <= sc.t_lt_h) { c[3]++; su[3] += csum(id); }
+                }
+            }
+        }
+        // boundary candidates of the low box
+        {
+            Box b = scan_box(sc, false, xi, sc.h);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                row_run(d.bstart, d.bend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1,
+                        st, en);
+                for (int j = st; j < en; ++j) {
+                    c[4]++;
+                    su[4] += csum(d.bids[j]);
+                }
+            }
+        }
+        for (int q = 0; q < 5; ++q) {
+            counts[5 * t + q] = c[q];
+            sums[5 * t + q] = su[q];
+        }
+    }
+}
+
+// Stable removal of one scene's entries from the current lists (used when a
+// scene is retired between steps). Single block, serial-in-tile order.
+__global__ __launch_bounds__(1024) void k_filter_lists(DevState d, int scene,
+                                                      int* __restrict__ ll,
+                                                      int* __restrict__ lh,
+                                                      int* __restrict__ lg) {
+    __shared__ int part[1024];
+    __shared__ int base;
+    int* lists[3] = {ll, lh, lg};
+    int* ns[3] = {&d.counts->n_l, &d.counts->n_h, &d.counts->n_g};
+    for (int which = 0; which < 3; ++which) {
+        int* L = lists[which];
+        const int n = *ns[which];
+        if (threadIdx.x == 0) base = 0;
+        __syncthreads();
+        for (int b0 = 0; b0 < n; b0 += 1024) {
+            int i = b0 + threadIdx.x;
+            int v = 0, keep = 0;
+            if (i < n) {
+                v = L[i];
+                int lid = which == 0 ? v : (which == 1 ? (v - d.CL) >> 2 : v);
+                keep = d.scene_of[lid] != scene;
+            }
+            part[threadIdx.x] = keep;
+            __syncthreads();
+            for (int off = 1; off < 1024; off <<= 1) {
+                int a = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+                __syncthreads();
+                part[threadIdx.x] += a;
+                __syncthreads();
+            }
+            int pos = base + part[threadIdx.x] - keep;
+            int tot = part[1023];
+            __syncthreads();
+            // in-place is safe: pos <= i and every read of this chunk
+            // happened before the barriers above
+            if (keep) L[pos] = v;
+            __syncthreads();
+            if (threadIdx.x == 0) base += tot;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *ns[which] = base;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+static inline int blocks_for(long long n, int bs, int cap) {
+    long long b = (n + bs - 1) / bs;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+void launch_fill_int(int* p, int value, long long n, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    k_fill_int<<<blocks_for(n, 256, 4096), 256, 0, st>>>(p, value, n);
+}
+
+// Grid build: clears, hash, sort, ranges + reorder. Returns the buffer index
+// holding the sorted keys/vals.
+static int enqueue_grid(const DevState& d, const StepShape& sh,
+                        hipStream_t st) {
+    const int par = sh.list_parity;
+    const long long ncells = (long long)d.CCL + d.CCH;
+    k_step_begin<<<blocks_for(ncells, 256, 4096), 256, 0, st>>>(d, ncells);
+    k_hash<<<sh.grid_elem, 256, 0, st>>>(d, d.list_l[par], d.list_h[par],
+                                         d.keys[0], d.vals[0]);
+    int cur = 0;
+    for (int pass = 0; pass < sh.passes; ++pass) {
+        const int shift = 8 * pass;
+        k_sort_hist<<<sh.sort_tiles, kSortThreads, 0, st>>>(
+            d.keys[cur], d.counts, shift, d.hist, sh.sort_tiles);
+        k_sort_scan<<<256, 256, 0, st>>>(d.hist, d.counts, sh.sort_tiles,
+                                          d.digit_tot);
+        k_sort_scatter<<<sh.sort_tiles, kSortThreads, 0, st>>>(
+            d.keys[cur], d.vals[cur], d.keys[cur ^ 1], d.vals[cur ^ 1],
+            d.counts, shift, d.hist, d.digit_tot, sh.sort_tiles);
+        cur ^= 1;
+    }
+    k_reorder<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[cur], d.vals[cur]);
+    return cur;
+}
+
+void launch_grid_only(const DevState& d, const StepShape& sh, void* stream) {
+    enqueue_grid(d, sh, (hipStream_t)stream);
+}
+
+void launch_step(const DevState& d, const StepShape& sh, void* stream,
+                 void* timing_events) {
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t* ev = (hipEvent_t*)timing_events;
+    const int par = sh.list_parity;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    const int cur = enqueue_grid(d, sh, st);
+    const uint32_t* vals = d.vals[cur];
+    if (ev) (void)hipEventRecord(ev[1], st);
+    k_density<<<sh.grid_elem, sh.dens_block, 0, st>>>(d, vals);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    k_force<<<sh.grid_elem, sh.force_block, 0, st>>>(d, vals);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    k_reduce<<<dim3(kRedChunks, sh.S), 256, 0, st>>>(d, vals);
+    k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
+    k_groups<<<blocks_for(d.CL, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
+    k_cap<<<sh.S, 256, 0, st>>>(d);
+    const int ntiles = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
+    k_compact_reduce<<<ntiles, kScanThreads, 0, st>>>(d, sh, vals,
+                                                      d.list_g[par]);
+    k_compact_top<<<1, 1024, 0, st>>>(d, sh);
+    k_compact_scatter<<<ntiles, kScanThreads, 0, st>>>(
+        d, sh, vals, d.list_g[par], d.list_l[par ^ 1], d.list_h[par ^ 1],
+        d.list_g[par ^ 1]);
+    if (ev) (void)hipEventRecord(ev[4], st);
+    k_finalize<<<blocks_for(sh.S, 64, 1024), 64, 0, st>>>(d);
+    if (ev) (void)hipEventRecord(ev[5], st);
+}
+
+void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
+                         void* stream) {
+    const int par = sh.list_parity;
+    k_filter_lists<<<1, 1024, 0, (hipStream_t)stream>>>(
+        d, scene, d.list_l[par], d.list_h[par], d.list_g[par]);
+}
+
+// Neighbour inspection of the first n entries of a level's current list.
+void launch_neighbors_n(const DevState& d, const StepShape& sh, int scene,
+                        int level, int n, uint32_t* counts, uint32_t* sums,
+                        void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const int cur = enqueue_grid(d, sh, st);
+    const int par = sh.list_parity;
+    const int* list = level ? d.list_h[par] : d.list_l[par];
+    k_neighbors<<<blocks_for(n, 256, 4096), 256, 0, st>>>(d, d.vals[cur], list,
+                                                          n, scene, counts, sums);
+}
+
+}  // namespace tss

@@ -1,0 +1,267 @@
+"""Python handles over libtss: a single-scene ``Solver`` (the reference's
+WCSPHSolver, TwoScale2D/WCSPHSolver.h:37-63) and a multi-scene ``Batch``.
+
+Arrays cross the boundary as numpy host arrays; the device state stays
+resident in the library between calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+_DTYPES = {
+    "POS": (np.float32, 2), "VEL": (np.float32, 2), "ACC": (np.float32, 2),
+    "RHO": (np.float32, 1), "P": (np.float32, 1), "VIS": (np.float32, 1),
+    "STATE": (np.uint8, 1), "AGE": (np.int32, 1),
+}
+
+
+def _field_shape(name: str, cnt: dict) -> tuple[np.dtype, tuple]:
+    """dtype and shape of a field for the current counts."""
+    if name in ("DT",):
+        return np.float32, (1,)
+    if name == "POS_B":
+        return np.float32, (cnt["n_bnd"], 2)
+    if name == "BND_SORTED":
+        return np.int32, (cnt["n_bnd"],)
+    if name in ("BND_START", "BND_END", "CELL_START_L", "CELL_END_L"):
+        return np.int32, (cnt["cells_low"],)
+    if name in ("CELL_START_H", "CELL_END_H"):
+        return np.int32, (cnt["cells_high"],)
+    if name == "LIST_L":
+        return np.int32, (cnt["n_low"],)
+    if name == "LIST_H":
+        return np.int32, (cnt["n_high"],)
+    if name == "LIST_G":
+        return np.int32, (cnt["n_groups"],)
+    if name in ("SORTED_L", "KEYS_L"):
+        return np.int32, (cnt["n_low_last"],)
+    if name in ("SORTED_H", "KEYS_H"):
+        return np.int32, (cnt["n_high_last"],)
+    base, lvl = name.rsplit("_", 1)
+    dt, comp = _DTYPES[base]
+    n = cnt["cap_low"] * (4 if lvl == "H" else 1)
+    return dt, ((n, comp) if comp > 1 else (n,))
+
+
+def config_init(xs, ys, xe, ye, h, H, rho0, B, c, alpha, sigma, dt
+                ) -> N.TssConfig:
+    """WCSPHConfig's constructor (TwoScale2D/WCSPHSolver.cu:2287-2332)."""
+    cfg = N.TssConfig()
+    N.check(N.lib().tss_config_init(C.byref(cfg), xs, ys, xe, ye, h, H, rho0,
+                                    B, c, alpha, sigma, dt),
+            "tss_config_init")
+    return cfg
+
+
+class Solver:
+    """One scene resident on the GPU."""
+
+    def __init__(self, cfg: N.TssConfig, low_xy: np.ndarray, cap_low: int,
+                 bnd_xy: np.ndarray):
+        lib = N.lib()
+        low = np.ascontiguousarray(low_xy, dtype=np.float32).reshape(-1, 2)
+        bnd = np.ascontiguousarray(bnd_xy, dtype=np.float32).reshape(-1, 2)
+        h = C.c_void_p()
+        N.check(lib.tss_create(C.byref(cfg), N.ptr(low), len(low), int(cap_low),
+                               N.ptr(bnd), len(bnd), C.byref(h)), "tss_create")
+        self._h = h
+        self.cfg = cfg
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            N.lib().tss_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- stepping ----------------------------------------------------------
+    def step(self, n: int = 1) -> None:
+        N.check(N.lib().tss_step(self._h, n), "tss_step")
+
+    def step_graph(self, n: int = 1) -> None:
+        N.check(N.lib().tss_step_graph(self._h, n), "tss_step_graph")
+
+    def synchronize(self) -> None:
+        N.check(N.lib().tss_synchronize(self._h), "tss_synchronize")
+
+    # -- state -------------------------------------------------------------
+    def counts(self) -> dict:
+        c = N.TssCounts()
+        N.check(N.lib().tss_counts(self._h, C.byref(c)), "tss_counts")
+        return c.as_dict()
+
+    def read(self, name: str, cnt: dict | None = None) -> np.ndarray:
+        cnt = cnt or self.counts()
+        dt, shape = _field_shape(name, cnt)
+        out = np.zeros(shape, dtype=dt)
+        N.check(N.lib().tss_read(self._h, N.FIELD[name], N.ptr(out),
+                                 out.nbytes), f"tss_read({name})")
+        return out
+
+    def write(self, name: str, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        N.check(N.lib().tss_write(self._h, N.FIELD[name], N.ptr(a), a.nbytes),
+                f"tss_write({name})")
+
+    def set_counts(self, n_low: int, n_high: int, n_groups: int) -> None:
+        N.check(N.lib().tss_set_counts(self._h, n_low, n_high, n_groups),
+                "tss_set_counts")
+
+    def device_ptr(self, name: str) -> int:
+        p = C.c_void_p()
+        N.check(N.lib().tss_device_ptr(self._h, N.FIELD[name], C.byref(p)),
+                "tss_device_ptr")
+        return p.value
+
+    def conservation(self) -> tuple[np.ndarray, np.ndarray]:
+        out = np.zeros(5, np.float64)
+        err = np.zeros(5, np.float64)
+        N.check(N.lib().tss_conservation(self._h, N.ptr(out), N.ptr(err)),
+                "tss_conservation")
+        return out, err
+
+    def neighbors(self, level: int) -> tuple[np.ndarray, np.ndarray]:
+        cnt = self.counts()
+        n = cnt["n_high"] if level else cnt["n_low"]
+        c = np.zeros((n, 5), np.uint32)
+        s = np.zeros((n, 5), np.uint32)
+        N.check(N.lib().tss_neighbors(self._h, level, N.ptr(c), N.ptr(s), n),
+                "tss_neighbors")
+        return c, s
+
+    def enable_timing(self, on: bool = True) -> None:
+        N.check(N.lib().tss_enable_timing(self._h, 1 if on else 0),
+                "tss_enable_timing")
+
+    def phase_times(self) -> np.ndarray:
+        out = np.zeros(5, np.float32)
+        N.check(N.lib().tss_phase_times(self._h, N.ptr(out)),
+                "tss_phase_times")
+        return out
+
+    def snapshot(self) -> dict:
+        """Every by-id field, the lists and dt (synchronising)."""
+        cnt = self.counts()
+        names = ["POS_L", "VEL_L", "ACC_L", "RHO_L", "P_L", "STATE_L", "AGE_L",
+                 "POS_H", "VEL_H", "ACC_H", "RHO_H", "P_H", "STATE_H",
+                 "LIST_L", "LIST_H", "LIST_G", "DT"]
+        snap = {n: self.read(n, cnt) for n in names}
+        snap["counts"] = cnt
+        return snap
+
+    def restore(self, snap: dict) -> None:
+        """Re-seed the solver with a snapshot taken from a solver of the
+        same configuration."""
+        for n in ["POS_L", "VEL_L", "ACC_L", "RHO_L", "P_L", "STATE_L",
+                  "AGE_L", "POS_H", "VEL_H", "ACC_H", "RHO_H", "P_H",
+                  "STATE_H", "LIST_L", "LIST_H", "LIST_G", "DT"]:
+            self.write(n, snap[n])
+        c = snap["counts"]
+        self.set_counts(c["n_low"], c["n_high"], c["n_groups"])
+
+
+class Batch:
+    """Many independent scenes advanced by one launch per phase."""
+
+    def __init__(self, max_scenes: int, cap_low: int, cap_bnd: int,
+                 cells_low: int, cells_high: int):
+        self.desc = N.TssBatchDesc(max_scenes, cap_low, cap_bnd, cells_low,
+                                   cells_high)
+        h = C.c_void_p()
+        N.check(N.lib().tss_batch_create(C.byref(self.desc), C.byref(h)),
+                "tss_batch_create")
+        self._h = h
+
+    @staticmethod
+    def memory_bytes(max_scenes, cap_low, cap_bnd, cells_low, cells_high
+                     ) -> int:
+        d = N.TssBatchDesc(max_scenes, cap_low, cap_bnd, cells_low, cells_high)
+        b = C.c_longlong()
+        N.check(N.lib().tss_batch_memory_bytes(C.byref(d), C.byref(b)),
+                "tss_batch_memory_bytes")
+        return b.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            N.lib().tss_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, cfg: N.TssConfig, low_xy: np.ndarray, cap_low: int,
+                  bnd_xy: np.ndarray) -> int:
+        low = np.ascontiguousarray(low_xy, dtype=np.float32).reshape(-1, 2)
+        bnd = np.ascontiguousarray(bnd_xy, dtype=np.float32).reshape(-1, 2)
+        sid = C.c_int()
+        N.check(N.lib().tss_batch_add_scene(self._h, C.byref(cfg), N.ptr(low),
+                                            len(low), int(cap_low), N.ptr(bnd),
+                                            len(bnd), C.byref(sid)),
+                "tss_batch_add_scene")
+        return sid.value
+
+    def remove_scene(self, sid: int) -> None:
+        N.check(N.lib().tss_batch_remove_scene(self._h, sid),
+                "tss_batch_remove_scene")
+
+    def step(self, n: int = 1) -> None:
+        N.check(N.lib().tss_batch_step(self._h, n), "tss_batch_step")
+
+    def step_graph(self, n: int = 1) -> None:
+        N.check(N.lib().tss_batch_step_graph(self._h, n),
+                "tss_batch_step_graph")
+
+    def synchronize(self) -> None:
+        N.check(N.lib().tss_batch_synchronize(self._h), "tss_batch_synchronize")
+
+    def counts(self, sid: int) -> dict:
+        c = N.TssCounts()
+        N.check(N.lib().tss_batch_counts(self._h, sid, C.byref(c)),
+                "tss_batch_counts")
+        return c.as_dict()
+
+    def read(self, sid: int, name: str, cnt: dict | None = None
+             ) -> np.ndarray:
+        cnt = cnt or self.counts(sid)
+        dt, shape = _field_shape(name, cnt)
+        out = np.zeros(shape, dtype=dt)
+        N.check(N.lib().tss_batch_read(self._h, sid, N.FIELD[name],
+                                       N.ptr(out), out.nbytes),
+                f"tss_batch_read({name})")
+        return out
+
+    def conservation(self, sid: int) -> tuple[np.ndarray, np.ndarray]:
+        out = np.zeros(5, np.float64)
+        err = np.zeros(5, np.float64)
+        N.check(N.lib().tss_batch_conservation(self._h, sid, N.ptr(out),
+                                               N.ptr(err)),
+                "tss_batch_conservation")
+        return out, err
+
+    def num_scenes(self) -> int:
+        n = C.c_int()
+        N.check(N.lib().tss_batch_num_scenes(self._h, C.byref(n)),
+                "tss_batch_num_scenes")
+        return n.value
+
+    def set_tuning(self, density_block: int, force_block: int) -> None:
+        N.check(N.lib().tss_batch_set_tuning(self._h, density_block,
+                                             force_block),
+                "tss_batch_set_tuning")
