@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Throughput of the two-scale WCSPH step on one MI355X.
+
+Workload (the headline "1M + 4M two-scale" configuration): large dam break,
+domain [0,4]x[0,1], 1414x707 = 999,698 low-res particles (dx = 0.8/707),
+every low particle split 1 -> 4 at the first step (forced all-split: the
+reference's split rule with the threshold left of the domain), giving
+999,698 low + 3,998,792 high-res particles advanced per step, CFL-0.4
+adaptive dt, FP32 with compensated sums, hipGraph-captured steps.
+
+A step advances every low and high particle once: grid build (hash, stable
+radix sort, cell ranges, reorder), density + Tait, forces + symplectic
+Euler, adaptivity + list compaction, FP64 reductions (dt, conservation).
+value = particle-steps/s = steps * (n_low + n_high) / wall time, with the
+state resident in HBM.
+
+Prints ONE JSON line. --gpus must be 1 (one GPU per box).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M+4M two-scale; p50/p99 step latency, batched scenes"
+
+# MI355X published peaks
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3          # published FP32 (vector, packed issue)
+FP32_NONPACKED_TFLOPS = 78.6      # without packed-FP32 issue
+
+# per-item flop weights of the neighbour kernels (candidate test, density
+# pair, force pair, wall pair)
+W_CAND, W_DENS, W_FORCE, W_WALL = 7, 28, 73, 20
+
+
+def _log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def _ensure_built():
+    from twoscalesplit_amd import _native
+    if not os.path.exists(_native.LIB_PATH):
+        from twoscalesplit_amd import build
+        build.build_lib()
+
+
+def grid_bytes(n: int, cells: int, passes: int) -> float:
+    """Algorithmic HBM bytes of one grid build: hash (read pos, write key and
+    index), P radix passes over key+value, range read, reorder gather +
+    scatter of the 21-byte payload, cell table clear + writes."""
+    S = 21
+    return n * (16 + 16 * passes + 4 + 2 * S) + 16 * cells
+
+
+def cpu_baseline(snap: dict, sc, steps: int, threads: int) -> dict:
+    """The oracle's FP32 transcription of the reference kernels (OpenMP)
+    on the GPU state of the same workload."""
+    from oracle import binding as ob
+    o = ob.Scene(ob.params_from_config(sc.cfg), sc.cap_low, sc.bnd_xy)
+    o.load_snapshot(snap)
+    n = o.s.n_l + o.s.n_h
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(32, threads)
+    dt = time.perf_counter() - t0
+    return {"value": steps * n / dt, "unit": "particle-steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {n}-particle state "
+                      f"(taken from the GPU after warmup), oracle FP32 "
+                      f"transcription of the reference kernels, OpenMP",
+            "seconds": round(dt, 3)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--latency-steps", type=int, default=60)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--natural", action="store_true",
+                    help="surface-band C3 instead of the forced all-split")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
+                          "sharding is design-only (--gpus must be 1)"}))
+        return 2
+    _ensure_built()
+    from twoscalesplit_amd import Solver, scenes
+
+    _log("scene")
+    sc = scenes.c3_large(forced=not args.natural)
+    sol = Solver(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    _log("warmup", args.warmup)
+    sol.step_graph(args.warmup)
+    sol.synchronize()
+    c0 = sol.counts()
+    n0 = c0["n_low"] + c0["n_high"]
+    _log("timed", args.steps, "steps of", n0, "particles")
+    t0 = time.perf_counter()
+    sol.step_graph(args.steps)
+    sol.synchronize()
+    t1 = time.perf_counter()
+    c1 = sol.counts()
+    elapsed = t1 - t0
+    # particle-steps: every step advances the lists it starts from; with the
+    # forced split the population is constant after the first step
+    n1 = c1["n_low_last"] + c1["n_high_last"]
+    psteps = args.steps * (n0 + n1) / 2.0
+    value = psteps / elapsed
+
+    # per-step latency (host wall clock, one synchronised step each)
+    _log("latency", args.latency_steps)
+    lat = []
+    for _ in range(args.latency_steps):
+        a = time.perf_counter()
+        sol.step(1)
+        sol.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    lat = np.array(lat)
+
+    # per-phase device time with hipEvents on the solver's stream
+    _log("phase timing")
+    sol.enable_timing(True)
+    sol.step(20)
+    ph = sol.phase_times()
+    sol.enable_timing(False)
+    cnt = sol.counts()
+    work = sol.work_counts()
+    cand = (work["cand_ll"] + work["cand_lh"] + work["cand_hh"] +
+            work["cand_hl"])
+    flops_dens = W_CAND * cand + W_DENS * work["dens_pairs"]
+    flops_force = (W_CAND * cand + W_FORCE * work["force_pairs"] +
+                   W_WALL * work["wall_cand"])
+    dens_tf = flops_dens / (ph[1] * 1e-3) / 1e12
+    force_tf = flops_force / (ph[2] * 1e-3) / 1e12
+    n_now = cnt["n_low"] + cnt["n_high"]
+    cells = cnt["cells_low"] + cnt["cells_high"]
+    passes = max(1, math.ceil(math.ceil(math.log2(max(cells, 2))) / 8))
+    gb = grid_bytes(n_now, cells, passes)
+    grid_gbs = gb / (ph[0] * 1e-3) / 1e9
+    if ph[2] >= ph[1]:
+        dom = ("force", force_tf, flops_force, ph[2])
+    else:
+        dom = ("density", dens_tf, flops_dens, ph[1])
+    roofline = {
+        "kernel": f"k_{dom[0]}",
+        "bound": "valu",
+        "achieved": round(dom[1], 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(dom[1] / FP32_PEAK_TFLOPS, 4),
+        "frac_nonpacked_peak": round(dom[1] / FP32_NONPACKED_TFLOPS, 4),
+        "traffic": None,
+        "algorithmic_flops_per_launch": int(dom[2]),
+        "avg_launch_ms": round(float(dom[3]), 4),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": sc.name,
+            "n_low": c1["n_low_last"],
+            "n_high": c1["n_high_last"],
+            "dt": "CFL 0.4 (FP64 max-reduction on the GPU)",
+            "step": "hipGraph-captured, device-resident counts",
+        },
+        "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                            "p99": round(float(np.percentile(lat, 99)), 4)},
+        "phase_ms": {"grid": round(float(ph[0]), 4),
+                     "density": round(float(ph[1]), 4),
+                     "force": round(float(ph[2]), 4),
+                     "adapt": round(float(ph[3]), 4),
+                     "reduce": round(float(ph[4]), 4)},
+        "density_tflops": round(dens_tf, 3),
+        "roofline": roofline,
+        "grid_roofline": {
+            "kernel": "grid build (hash + radix sort + ranges + reorder)",
+            "bound": "hbm",
+            "achieved": round(grid_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(grid_gbs / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": int(gb),
+            "radix_passes": passes,
+        },
+        "work": work,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        _log("cpu baseline", args.cpu_steps, "steps,", threads, "threads")
+        snap = sol.snapshot()
+        out["cpu_baseline"] = cpu_baseline(snap, sc, args.cpu_steps, threads)
+    else:
+        out["cpu_baseline"] = None
+    sol.close()
+    print(json.dumps(out, default=float), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -193,6 +193,14 @@ int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums,
  * [3] adapt+compaction [4] reductions. */
 int tss_enable_timing(tss_solver* s, int on);
 int tss_phase_times(tss_solver* s, float out_ms[5]);
+/* Exact work of the density and force kernels on the current state (grid
+ * rebuilt from the current lists): [0] low-target same-level candidates,
+ * [1] low-target cross-level candidates, [2] high-target same-level
+ * candidates, [3] high-target cross-level candidates, [4] density pairs,
+ * [5] force pairs, [6] wall candidates, [7] targets. Multiplied by fixed
+ * per-item flop weights these give the kernels' algorithmic flops. */
+#define TSS_WORK_COUNT 8
+int tss_work_counts(tss_solver* s, unsigned long long out[TSS_WORK_COUNT]);
 /* Replaces WCSPHSolver::~WCSPHSolver (R/WCSPHSolver.cu:2522-2535). */
 void tss_destroy(tss_solver* s);
 

@@ -101,6 +101,7 @@ PROTOTYPES = {
     "tss_neighbors": (_I, [_P, _I, _P, _P, _SZ]),
     "tss_enable_timing": (_I, [_P, _I]),
     "tss_phase_times": (_I, [_P, _P]),
+    "tss_work_counts": (_I, [_P, _P]),
     "tss_destroy": (None, [_P]),
     "tss_batch_memory_bytes": (_I, [C.POINTER(TssBatchDesc),
                                     C.POINTER(C.c_longlong)]),

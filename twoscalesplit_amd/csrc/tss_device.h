@@ -154,6 +154,8 @@ void launch_neighbors_n(const DevState& d, const StepShape& sh, int scene,
                         int level, int n, uint32_t* counts, uint32_t* sums,
                         void* stream);
 void launch_grid_only(const DevState& d, const StepShape& sh, void* stream);
+void launch_work_counts(const DevState& d, const StepShape& sh,
+                        unsigned long long* out, void* stream);
 void launch_fill_int(int* p, int value, long long n, void* stream);
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
                          void* stream);

@@ -598,27 +598,28 @@ struct tss_batch {
         update_shape();
     }
 
+    // With timing on, every step records hipEvents around its phases on the
+    // batch's stream and waits for them; phase times accumulate until the
+    // next tss_enable_timing call.
     void step_eager(int n) {
         for (int i = 0; i < n; ++i) {
             tss::launch_step(d, sh, stream, timing ? (void*)ev : nullptr);
             sh.list_parity ^= 1;
-        }
-        if (timing && n > 0) {
-            HIP_CHECK(hipEventSynchronize(ev[5]));
-            float t;
-            HIP_CHECK(hipEventElapsedTime(&t, ev[0], ev[1]));
-            phase_ms[0] = t;
-            HIP_CHECK(hipEventElapsedTime(&t, ev[1], ev[2]));
-            phase_ms[1] = t;
-            HIP_CHECK(hipEventElapsedTime(&t, ev[2], ev[3]));
-            phase_ms[2] = t;
-            HIP_CHECK(hipEventElapsedTime(&t, ev[3], ev[4]));
-            phase_ms[3] = t;
-            HIP_CHECK(hipEventElapsedTime(&t, ev[4], ev[5]));
-            phase_ms[4] = t;
+            if (timing) {
+                HIP_CHECK(hipEventSynchronize(ev[5]));
+                for (int p = 0; p < 5; ++p) {
+                    float t;
+                    HIP_CHECK(hipEventElapsedTime(&t, ev[p], ev[p + 1]));
+                    phase_sum[p] += t;
+                }
+                ++phase_n;
+            }
         }
         HIP_CHECK(hipGetLastError());
     }
+
+    double phase_sum[5] = {0, 0, 0, 0, 0};
+    long long phase_n = 0;
 
     // Two consecutive steps captured once per starting list parity and
     // replayed: no host synchronisation, counts stay on the device.
@@ -904,6 +905,22 @@ struct tss_batch {
         HIP_CHECK(hipGetLastError());
     }
 
+    void work_counts(int sid, unsigned long long* out) {
+        int alive = 0;
+        for (auto& x : scenes) alive += x.alive;
+        require(alive == 1 && scenes[sid].alive, "work counts need a batch of one scene");
+        unsigned long long* dev = nullptr;
+        HIP_CHECK(hipMalloc(&dev, 8 * sizeof(unsigned long long)));
+        tss::launch_work_counts(d, sh, dev, stream);
+        hipError_t e = hipMemcpyAsync(out, dev, 8 * sizeof(unsigned long long),
+                                      hipMemcpyDeviceToHost, stream);
+        hipError_t e2 = hipStreamSynchronize(stream);
+        (void)hipFree(dev);
+        HIP_CHECK(e);
+        HIP_CHECK(e2);
+        HIP_CHECK(hipGetLastError());
+    }
+
     void* device_ptr(int sid, int field) {
         FieldView v = view(sid, field);
         require(v.kind == 0, "field has no plain device view");
@@ -1113,13 +1130,23 @@ int tss_enable_timing(tss_solver* s, int on) {
     return guarded([&] {
         require(s, "null argument");
         s->b.timing = on != 0;
+        for (double& x : s->b.phase_sum) x = 0.0;
+        s->b.phase_n = 0;
     });
 }
 
 int tss_phase_times(tss_solver* s, float out[5]) {
     return guarded([&] {
         require(s && out, "null argument");
-        for (int i = 0; i < 5; ++i) out[i] = s->b.phase_ms[i];
+        for (int i = 0; i < 5; ++i)
+            out[i] = s->b.phase_n ? (float)(s->b.phase_sum[i] / (double)s->b.phase_n) : 0.0f;
+    });
+}
+
+int tss_work_counts(tss_solver* s, unsigned long long out[TSS_WORK_COUNT]) {
+    return guarded([&] {
+        require(s && out, "null argument");
+        s->b.work_counts(s->sid, out);
     });
 }
 

@@ -1322,6 +1322,73 @@ __global__ void k_neighbors(DevState d, const uint32_t* __restrict__ vals,
     }
 }
 
+// Exact work of the density and force kernels on the current grid (the
+// algorithmic flop counts of the roofline): candidates scanned per box kind,
+// pairs that pass each kernel's cutoff and weight, wall candidates, targets.
+__global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
+    const int nl = d.counts->n_l;
+    const int n = nl + d.counts->n_h;
+    unsigned long long w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    GRID_STRIDE(k, n) {
+        const int s = d.sscene[k];
+        const SceneDev& sc = d.scenes[s];
+        const float2 xi = d.spos[k];
+        const uint8_t si = d.sstate[k];
+        const bool high = k >= nl;
+        const bool i_prom = !(si & 1u);
+        w[7] += 1;
+        // same level
+        {
+            Box b = scan_box(sc, high, xi, high ? sc.H : sc.h);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                int row = high ? sc.cbase_h + cy * sc.dhx : sc.cbase_l + cy * sc.dlx;
+                row_run(d.cstart, d.cend, row, b.x0, b.x1, st, en);
+                for (int j = st; j < en; ++j) {
+                    float r2 = dist2(xi, d.spos[j]);
+                    const uint8_t sj = d.sstate[j];
+                    w[high ? 2 : 0] += 1;
+                    bool lam = high ? !(i_prom && (sj & 1u)) : true;
+                    if (lam && r2 <= (high ? sc.t_le_H : sc.t_le_h)) w[4] += 1;
+                    if (lam && r2 != 0.0f && r2 <= (high ? sc.t_lt_H : sc.t_lt_h)) w[5] += 1;
+                }
+            }
+        }
+        // cross level
+        if (high || d.scnt[s].seg_h1 > d.scnt[s].seg_h0) {
+            Box b = scan_box(sc, !high, xi, sc.h);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                int row = !high ? sc.cbase_h + cy * sc.dhx : sc.cbase_l + cy * sc.dlx;
+                row_run(d.cstart, d.cend, row, b.x0, b.x1, st, en);
+                for (int j = st; j < en; ++j) {
+                    float r2 = dist2(xi, d.spos[j]);
+                    const uint8_t sj = d.sstate[j];
+                    w[high ? 3 : 1] += 1;
+                    bool lam = high ? !((si & 1u) && (sj & 1u)) : !(sj & 1u);
+                    if (lam && r2 <= (high ? sc.t_le_H : sc.t_le_h)) w[4] += 1;
+                    if (lam && r2 != 0.0f && r2 <= sc.t_lt_h && (!high || sj != kDeleted))
+                        w[5] += 1;
+                }
+            }
+        }
+        // wall candidates of the force kernel
+        {
+            Box b = scan_box(sc, false, xi, sc.h);
+            for (int cy = b.y0; cy <= b.y1; ++cy) {
+                int st, en;
+                row_run(d.bstart, d.bend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1, st, en);
+                if (en > st) w[6] += (unsigned long long)(en - st);
+            }
+        }
+    }
+    for (int q = 0; q < 8; ++q) {
+        unsigned long long v = w[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&out[q], v);
+    }
+}
+
 // Stable removal of one scene's entries from the current lists (used when a
 // scene is retired between steps). Single block, serial-in-tile order.
 __global__ __launch_bounds__(1024) void k_filter_lists(DevState d, int scene,
@@ -1446,6 +1513,14 @@ void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
     const int par = sh.list_parity;
     k_filter_lists<<<1, 1024, 0, (hipStream_t)stream>>>(
         d, scene, d.list_l[par], d.list_h[par], d.list_g[par]);
+}
+
+void launch_work_counts(const DevState& d, const StepShape& sh,
+                        unsigned long long* out, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    enqueue_grid(d, sh, st);
+    (void)hipMemsetAsync(out, 0, 8 * sizeof(unsigned long long), st);
+    k_work<<<sh.grid_elem, 256, 0, st>>>(d, out);
 }
 
 // Neighbour inspection of the first n entries of a level's current list.

@@ -153,6 +153,14 @@ class Solver:
                 "tss_phase_times")
         return out
 
+    def work_counts(self) -> dict:
+        out = np.zeros(8, np.uint64)
+        N.check(N.lib().tss_work_counts(self._h, N.ptr(out)),
+                "tss_work_counts")
+        keys = ["cand_ll", "cand_lh", "cand_hh", "cand_hl", "dens_pairs",
+                "force_pairs", "wall_cand", "targets"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
     def snapshot(self) -> dict:
         """Every by-id field, the lists and dt (synchronising)."""
         cnt = self.counts()
