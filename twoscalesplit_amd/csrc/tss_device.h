@@ -28,7 +28,11 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort tile
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // entries per compaction tile
-constexpr int kRedChunks = 32;    // reduction chunks per scene
+// FP64 reductions: a scene's sorted particles are cut into chunks of
+// kRedChunk; a scene owns ceil(5 cap_low / kRedChunk) chunk slots (from its
+// own capacity, so the partition -- and every rounding -- is the same alone
+// and in a batch).
+constexpr int kRedChunk = 4096;
 // mass, px, py, ke, pe, max|v|^2, sum|px|, sum|py|, sum|pe|
 constexpr int kRedFields = 9;
 
@@ -131,8 +135,12 @@ struct DevState {
     int* tile_sums;           // [2 * max_scan_tiles]
     int* tile_base;           // [2 * max_scan_tiles]
     int* seg_base;            // [8]
-    double* red;              // [S * kRedChunks * kRedFields]
+    double* red;              // [red_slots * kRedFields]
+    int* red_scene;           // [red_slots] scene of each chunk slot (-1 none)
+    int* red_base;            // [S] first chunk slot of each scene
+    int* red_n;               // [S] chunk slots of each scene
     double* cons;             // [S * 10] conservation sums + bounds
+    uint8_t* cflag;           // [2 (CCL + CCH)] cflag[2c + s] = 1: state s in cell c
 };
 
 // Host-known launch parameters of one step.
@@ -144,6 +152,7 @@ struct StepShape {
     int scan_tiles_l, scan_tiles_h, scan_tiles_g;  // capacity-based tiles
     int dens_block, force_block;
     int S;
+    int red_slots;            // chunk slots in use (grid of the reduction)
     int list_parity;          // which list buffer holds the current lists
 };
 

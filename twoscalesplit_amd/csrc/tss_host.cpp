@@ -228,8 +228,9 @@ struct tss_batch {
         b += ds.cap_bnd * 12;
         b += U * (8 + 8 + 4 + 4 + 1 + 1 + 4 + 4);  // sorted copies + sidx
         b += CL * 2;
-        b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) +
-                                         (tss::kRedChunks * tss::kRedFields + 10) * 8);
+        b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) + 10 * 8 + 12);
+        b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
+        b += 2 * (ds.cells_low + ds.cells_high);  // cell state flags
         return b;
     }
 
@@ -297,14 +298,20 @@ struct tss_batch {
         d.tile_sums = dalloc<int>(2 * (size_t)nst, owned);
         d.tile_base = dalloc<int>(2 * (size_t)nst, owned);
         d.seg_base = dalloc<int>(8, owned);
-        d.red = dalloc<double>((size_t)max_scenes * tss::kRedChunks * tss::kRedFields, owned);
+        red_cap = (int)((U + tss::kRedChunk - 1) / tss::kRedChunk) + max_scenes + 1;
+        d.red = dalloc<double>((size_t)red_cap * tss::kRedFields, owned);
+        d.red_scene = dalloc<int>(red_cap, owned);
+        d.red_base = dalloc<int>(max_scenes, owned);
+        d.red_n = dalloc<int>(max_scenes, owned);
         d.cons = dalloc<double>((size_t)max_scenes * 10, owned);
+        d.cflag = dalloc<uint8_t>(2 * (ccl + cch), owned);
         // initial contents
         HIP_CHECK(hipMemsetAsync(d.scenes, 0, sizeof(tss::SceneDev) * max_scenes, stream));
         HIP_CHECK(hipMemsetAsync(d.scnt, 0, sizeof(tss::SceneCount) * max_scenes, stream));
         HIP_CHECK(hipMemsetAsync(d.counts, 0, sizeof(tss::DevCounts), stream));
         HIP_CHECK(hipMemsetAsync(d.cons, 0, sizeof(double) * 10 * max_scenes, stream));
-        HIP_CHECK(hipMemsetAsync(d.red, 0, sizeof(double) * tss::kRedChunks * tss::kRedFields * max_scenes, stream));
+        HIP_CHECK(hipMemsetAsync(d.red, 0, sizeof(double) * tss::kRedFields * red_cap, stream));
+        HIP_CHECK(hipMemsetAsync(d.cflag, 0, 2 * (size_t)(ccl + cch), stream));
         tss::launch_fill_int(d.scene_of, 0, CL, stream);
         HIP_CHECK(hipMemsetAsync(d.state, 0, (size_t)U, stream));
         HIP_CHECK(hipMemsetAsync(d.pos, 0, sizeof(float2) * U, stream));
@@ -334,7 +341,34 @@ struct tss_batch {
         update_shape();
     }
 
+    // Reduction chunk slots: each live scene owns ceil(5 cap_low / chunk)
+    // consecutive slots (a function of its own capacity only).
+    void upload_reduction_tables() {
+        std::vector<int> scene_of_slot, base(max_scenes, 0), n(max_scenes, 0);
+        for (int s = 0; s < max_scenes; ++s) {
+            if (!scenes[s].alive) continue;
+            const long long u = 5LL * scenes[s].lcap;
+            const int k = (int)((u + tss::kRedChunk - 1) / tss::kRedChunk);
+            base[s] = (int)scene_of_slot.size();
+            n[s] = k;
+            for (int i = 0; i < k; ++i) scene_of_slot.push_back(s);
+        }
+        if (scene_of_slot.empty()) scene_of_slot.push_back(-1);
+        require((int)scene_of_slot.size() <= red_cap, "reduction slots exhausted");
+        sh.red_slots = (int)scene_of_slot.size();
+        HIP_CHECK(hipMemcpyAsync(d.red_scene, scene_of_slot.data(), 4 * scene_of_slot.size(),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d.red_base, base.data(), 4 * (size_t)max_scenes,
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d.red_n, n.data(), 4 * (size_t)max_scenes,
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    int red_cap = 0;
+
     void update_shape() {
+        upload_reduction_tables();
         long long maxkey = 1;
         for (auto& s : scenes)
             if (s.alive) maxkey = std::max<long long>(maxkey, (long long)s.cbh + s.nch);

@@ -187,6 +187,8 @@ __global__ void k_step_begin(DevState d, long long ncells) {
          i < ncells; i += (long long)gridDim.x * blockDim.x) {
         d.cstart[i] = -1;
         d.cend[i] = -1;
+        d.cflag[2 * i] = 0;
+        d.cflag[2 * i + 1] = 0;
     }
     GRID_STRIDE(s, d.S) {
         SceneCount& c = d.scnt[s];
@@ -376,6 +378,9 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
         const int u = (int)vals[k];
         if (k == 0 || keys[k - 1] != key) d.cstart[key] = k;
         if (k == n - 1 || keys[k + 1] != key) d.cend[key] = k + 1;
+        // states present per cell (every particle stores the same 1 for its
+        // state): lets the scans skip rows whose weights are all zero
+        d.cflag[2 * (size_t)key + (d.state[u] & 1u)] = 1;
         d.spos[k] = d.pos[u];
         d.svel[k] = d.vel[u];
         d.sstate[k] = d.state[u];
@@ -405,320 +410,7 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
     }
 }
 
-// ---------------------------------------------------------------------------
-// Density + Tait pressure (R:335-474, 1342-1516; high<-low reads the low
-// arrays into its own sum: the method's ruling on R:1494-1504).
-// ---------------------------------------------------------------------------
-__global__ void k_density(DevState d, const uint32_t* __restrict__ vals) {
-    const int nl = d.counts->n_l;
-    const int n = nl + d.counts->n_h;
-    const int* __restrict__ cs = d.cstart;
-    const int* __restrict__ ce = d.cend;
-    const float2* __restrict__ spos = d.spos;
-    const uint8_t* __restrict__ sst = d.sstate;
-    GRID_STRIDE(k, n) {
-        const int s = d.sscene[k];
-        const SceneDev& sc = d.scenes[s];
-        const float2 xi = spos[k];
-        const uint8_t si = sst[k];
-        const bool high = k >= nl;
-        float s1 = 0.f, c1 = 0.f, s2 = 0.f, c2 = 0.f;
-        if (!high) {
-            // low <- low, r <= h, every state (R:335-363)
-            Box b = scan_box(sc, false, xi, sc.h);
-            for (int cy = b.y0; cy <= b.y1; ++cy) {
-                int st, en;
-                row_run(cs, ce, sc.cbase_l + cy * sc.dlx, b.x0, b.x1, st, en);
-                for (int j = st; j < en; ++j) {
-                    float r2 = dist2(xi, spos[j]);
-                    if (r2 <= sc.t_le_h) {
-                        float r = dist_from2(r2);
-                        float w, g;
-                        m4_blend(sc, r, w, g);
-                        kahan_add(s1, c1, w);
-                    }
-                }
-            }
-            // low <- high, r <= h, lambda_LH (R:365-400)
-            if (sc.dhx > 0) {
-                Box bh = scan_box(sc, true, xi, sc.h);
-                for (int cy = bh.y0; cy <= bh.y1; ++cy) {
-                    int st, en;
-                    row_run(cs, ce, sc.cbase_h + cy * sc.dhx, bh.x0, bh.x1, st,
-                            en);
-                    for (int j = st; j < en; ++j) {
-                        float r2 = dist2(xi, spos[j]);
-                        if (r2 <= sc.t_le_h && !(sst[j] & 1u)) {
-                            float r = dist_from2(r2);
-                            float w, g;
-                            m4_blend(sc, r, w, g);
-                            kahan_add(s2, c2, w);
-                        }
-                    }
-                }
-            }
-        } else {
-            // high <- high, r <= H, lambda_HH (R:402-437)
-            Box b = scan_box(sc, true, xi, sc.H);
-            const bool i_prom = !(si & 1u);
-            for (int cy = b.y0; cy <= b.y1; ++cy) {
-                int st, en;
-                row_run(cs, ce, sc.cbase_h + cy * sc.dhx, b.x0, b.x1, st, en);
-                for (int j = st; j < en; ++j) {
-                    float r2 = dist2(xi, spos[j]);
-                    if (r2 <= sc.t_le_H && !(i_prom && (sst[j] & 1u))) {
-                        float r = dist_from2(r2);
-                        float w, g;
-                        m4_blend(sc, r, w, g);
-                        kahan_add(s1, c1, w);
-                    }
-                }
-            }
-            // high <- low, r <= H, lambda_HL (R:439-474, low arrays)
-            Box bl = scan_box(sc, false, xi, sc.h);
-            for (int cy = bl.y0; cy <= bl.y1; ++cy) {
-                int st, en;
-                row_run(cs, ce, sc.cbase_l + cy * sc.dlx, bl.x0, bl.x1, st, en);
-                for (int j = st; j < en; ++j) {
-                    float r2 = dist2(xi, spos[j]);
-                    if (r2 <= sc.t_le_H && !((si & 1u) && (sst[j] & 1u))) {
-                        float r = dist_from2(r2);
-                        float w, g;
-                        m4_blend(sc, r, w, g);
-                        kahan_add(s2, c2, w);
-                    }
-                }
-            }
-        }
-        float hi, lo;
-        if (!high)
-            mass_sum(sc.m_l, s1, c1, sc.m_h, s2, c2, hi, lo);
-        else
-            mass_sum(sc.m_h, s1, c1, sc.m_l, s2, c2, hi, lo);
-        const float rho = hi + lo;
-        const float pr = tait(sc, hi, lo);
-        const int u = (int)vals[k];
-        d.srho[k] = rho;
-        d.spr[k] = pr / (rho * rho);
-        d.rho[u] = rho;
-        d.p[u] = pr;
-        // read-back quantity of setQuantities (R:1030-1051)
-        d.vis[u] = fminf(fabsf(rho - sc.rho0), 400.0f) / 400.0f;
-        // free-surface mark: density deficit, listed low or promoted high,
-        // and no wall particle within h (walls carry no density)
-        uint8_t surf = high ? (!(si & 1u) && rho < sc.surf_t_h)
-                            : (rho < sc.surf_t);
-        if (surf) {
-            Box b = scan_box(sc, false, xi, sc.h);
-            for (int cy = b.y0; cy <= b.y1 && surf; ++cy) {
-                int bs, be;
-                row_run(d.bstart, d.bend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1,
-                        bs, be);
-                for (int j = bs; j < be; ++j) {
-                    if (dist2(xi, d.bpos[j]) <= sc.t_le_h) {
-                        surf = 0;
-                        break;
-                    }
-                }
-            }
-        }
-        d.ssurf[k] = surf;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Forces + symplectic Euler (R:476-885, 1604-1646, 1648-1865, 1867-2033).
-// All terms read the cell-sorted pre-step copies; results go to the by-id
-// arrays, so there is no read/write race between particles.
-// ---------------------------------------------------------------------------
-struct Acc2 {
-    float s_x, c_x, s_y, c_y;  // compensated totals
-};
-
-__device__ __forceinline__ void acc_flush(Acc2& a, float& rx, float& ry) {
-    kahan_add(a.s_x, a.c_x, rx);
-    kahan_add(a.s_y, a.c_y, ry);
-    rx = 0.f;
-    ry = 0.f;
-}
-
-__device__ __forceinline__ float acc_get_x(const Acc2& a) { return a.s_x - a.c_x; }
-__device__ __forceinline__ float acc_get_y(const Acc2& a) { return a.s_y - a.c_y; }
-
-// One fluid pair (R:502-579 and its level variants): pressure + Monaghan
-// viscosity times grad W_bar, and the tension kernel term.
-__device__ __forceinline__ void fluid_pair(const SceneDev& sc, float2 xi,
-                                           float2 vi, float rhoi, float pri,
-                                           float2 xj, float2 vj, float rhoj,
-                                           float prj, float r2, float visc,
-                                           float eps, float& ax, float& ay,
-                                           float& tx, float& ty) {
-    float dx = xi.x - xj.x;
-    float dy = xi.y - xj.y;
-    float rinv = fast_rsq(r2);
-    float r = r2 * rinv;
-    float w, g;
-    m4_blend(sc, r, w, g);
-    float coeff = pri + prj;
-    float dvp = fmaf(vi.x - vj.x, dx, (vi.y - vj.y) * dy);
-    if (dvp < 0.0f) {
-        coeff -= dvp * visc * fast_rcp((r2 + eps) * (rhoi + rhoj));
-    }
-    float t = coeff * g * rinv;
-    ax = fmaf(t, dx, ax);
-    ay = fmaf(t, dy, ay);
-    tx = fmaf(w, dx, tx);
-    ty = fmaf(w, dy, ty);
-}
-
-__global__ void k_force(DevState d, const uint32_t* __restrict__ vals) {
-    const int nl = d.counts->n_l;
-    const int n = nl + d.counts->n_h;
-    const int* __restrict__ cs = d.cstart;
-    const int* __restrict__ ce = d.cend;
-    const float2* __restrict__ spos = d.spos;
-    const float2* __restrict__ svel = d.svel;
-    const float* __restrict__ srho = d.srho;
-    const float* __restrict__ spr = d.spr;
-    const uint8_t* __restrict__ sst = d.sstate;
-    GRID_STRIDE(k, n) {
-        const int s = d.sscene[k];
-        const SceneDev& sc = d.scenes[s];
-        const float2 xi = spos[k];
-        const float2 vi = svel[k];
-        const float rhoi = srho[k];
-        const float pri = spr[k];
-        const uint8_t si = sst[k];
-        const bool high = k >= nl;
-        Acc2 a1 = {0.f, 0.f, 0.f, 0.f};  // same-level pressure/viscosity
-        Acc2 a2 = {0.f, 0.f, 0.f, 0.f};  // cross-level
-        Acc2 at = {0.f, 0.f, 0.f, 0.f};  // tension
-        float bx = 0.f, by = 0.f;        // wall
-        float rx, ry, tx, ty;
-        if (!high) {
-            Box b = scan_box(sc, false, xi, sc.h);
-            for (int cy = b.y0; cy <= b.y1; ++cy) {
-                const int row = sc.cbase_l + cy * sc.dlx;
-                int st, en;
-                row_run(cs, ce, row, b.x0, b.x1, st, en);
-                rx = ry = tx = ty = 0.f;
-                for (int j = st; j < en; ++j) {
-                    float2 xj = spos[j];
-                    float r2 = dist2(xi, xj);
-                    if (r2 != 0.0f && r2 <= sc.t_lt_h) {
-                        // deleted (state 2) particles are never listed, so
-                        // the reference's deleted[] weight is 1 here
-                        fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j], srho[j],
-                                   spr[j], r2, sc.visc_h, sc.eps_h, rx, ry, tx,
-                                   ty);
-                    }
-                }
-                acc_flush(a1, rx, ry);
-                acc_flush(at, tx, ty);
-                // wall particles of the same low cells (R:565-579)
-                int bs, be;
-                row_run(d.bstart, d.bend, row, b.x0, b.x1, bs, be);
-                for (int j = bs; j < be; ++j) {
-                    float2 xb = d.bpos[j];
-                    float dx = xi.x - xb.x, dy = xi.y - xb.y;
-                    float r2 = dist2(xi, xb);
-                    float r = dist_from2(r2);
-                    float gm = sc.cb_l * wall_gamma(sc, r2, r, sc.q_h);
-                    bx = fmaf(gm, dx, bx);
-                    by = fmaf(gm, dy, by);
-                }
-            }
-            if (d.scnt[s].seg_h1 > d.scnt[s].seg_h0) {
-                Box bh = scan_box(sc, true, xi, sc.h);
-                for (int cy = bh.y0; cy <= bh.y1; ++cy) {
-                    int st, en;
-                    row_run(cs, ce, sc.cbase_h + cy * sc.dhx, bh.x0, bh.x1, st,
-                            en);
-                    rx = ry = tx = ty = 0.f;
-                    for (int j = st; j < en; ++j) {
-                        float2 xj = spos[j];
-                        float r2 = dist2(xi, xj);
-                        if (r2 != 0.0f && r2 <= sc.t_lt_h && !(sst[j] & 1u)) {
-                            fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j],
-                                       srho[j], spr[j], r2, sc.visc_x,
-                                       sc.eps_x, rx, ry, tx, ty);
-                        }
-                    }
-                    acc_flush(a2, rx, ry);
-                    acc_flush(at, tx, ty);
-                }
-            }
-        } else {
-            const bool i_prom = !(si & 1u);
-            Box b = scan_box(sc, true, xi, sc.H);
-            for (int cy = b.y0; cy <= b.y1; ++cy) {
-                int st, en;
-                row_run(cs, ce, sc.cbase_h + cy * sc.dhx, b.x0, b.x1, st, en);
-                rx = ry = tx = ty = 0.f;
-                for (int j = st; j < en; ++j) {
-                    float2 xj = spos[j];
-                    float r2 = dist2(xi, xj);
-                    if (r2 != 0.0f && r2 <= sc.t_lt_H &&
-                        !(i_prom && (sst[j] & 1u))) {
-                        fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j], srho[j],
-                                   spr[j], r2, sc.visc_H, sc.eps_H, rx, ry, tx,
-                                   ty);
-                    }
-                }
-                acc_flush(a1, rx, ry);
-                acc_flush(at, tx, ty);
-            }
-            Box bl = scan_box(sc, false, xi, sc.h);
-            for (int cy = bl.y0; cy <= bl.y1; ++cy) {
-                const int row = sc.cbase_l + cy * sc.dlx;
-                int st, en;
-                row_run(cs, ce, row, bl.x0, bl.x1, st, en);
-                rx = ry = tx = ty = 0.f;
-                for (int j = st; j < en; ++j) {
-                    float2 xj = spos[j];
-                    float r2 = dist2(xi, xj);
-                    const uint8_t sj = sst[j];
-                    // lambda_HL * deleted[s_j] (R:808-809)
-                    if (r2 != 0.0f && r2 <= sc.t_lt_h &&
-                        !((si & 1u) && (sj & 1u)) && sj != kDeleted) {
-                        fluid_pair(sc, xi, vi, rhoi, pri, xj, svel[j], srho[j],
-                                   spr[j], r2, sc.visc_x, sc.eps_x, rx, ry, tx,
-                                   ty);
-                    }
-                }
-                acc_flush(a2, rx, ry);
-                acc_flush(at, tx, ty);
-                int bs, be;
-                row_run(d.bstart, d.bend, row, bl.x0, bl.x1, bs, be);
-                for (int j = bs; j < be; ++j) {
-                    float2 xb = d.bpos[j];
-                    float dx = xi.x - xb.x, dy = xi.y - xb.y;
-                    float r2 = dist2(xi, xb);
-                    float r = dist_from2(r2);
-                    float gm = sc.cb_h * wall_gamma(sc, r2, r, sc.q_H);
-                    bx = fmaf(gm, dx, bx);
-                    by = fmaf(gm, dy, by);
-                }
-            }
-        }
-        // R:1790-1805 (low) / R:1997-2012 (high)
-        const float m_same = high ? sc.m_h : sc.m_l;
-        const float m_cross = high ? sc.m_l : sc.m_h;
-        float ax = -m_same * acc_get_x(a1) - m_cross * acc_get_x(a2);
-        float ay = -m_same * acc_get_y(a1) - m_cross * acc_get_y(a2);
-        ax = fmaf(-sc.sigma, acc_get_x(at), ax) + bx;
-        ay = fmaf(-sc.sigma, acc_get_y(at), ay) + by - sc.g;
-        const int u = (int)vals[k];
-        const float dt = sc.dt;
-        float vx = vi.x + dt * ax;
-        float vy = vi.y + dt * ay;
-        float px = xi.x + dt * vx;
-        float py = xi.y + dt * vy;
-        d.acc[u] = make_float2(ax, ay);
-        d.vel[u] = make_float2(vx, vy);
-        d.pos[u] = make_float2(px, py);
-    }
-}
+#include "tss_pairs.inc"
 
 // ---------------------------------------------------------------------------
 // FP64 reductions per scene: conservation sums over real particles (listed
@@ -728,16 +420,18 @@ __global__ void k_force(DevState d, const uint32_t* __restrict__ vals) {
 __global__ __launch_bounds__(256) void k_reduce(DevState d,
                                                 const uint32_t* __restrict__ vals) {
     __shared__ double sh[kRedFields][256];
-    const int s = blockIdx.y;
-    const int chunk = blockIdx.x;
+    const int slot = blockIdx.x;
+    const int s = d.red_scene[slot];
+    if (s < 0) return;  // whole block: slot not owned by any scene
+    const int chunk = slot - d.red_base[s];
     const SceneDev& sc = d.scenes[s];
     const SceneCount& c = d.scnt[s];
     const int nl_s = c.seg_l1 - c.seg_l0;
     const int m = nl_s + (c.seg_h1 - c.seg_h0);
     double acc[kRedFields] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (sc.alive && m > 0) {
-        const long long i0 = (long long)m * chunk / kRedChunks;
-        const long long i1 = (long long)m * (chunk + 1) / kRedChunks;
+        const long long i0 = (long long)chunk * kRedChunk;
+        const long long i1 = i0 + kRedChunk < m ? i0 + kRedChunk : (long long)m;
         const double g = (double)sc.g;
         for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
             int k = i < nl_s ? c.seg_l0 + (int)i : c.seg_h0 + (int)(i - nl_s);
@@ -778,7 +472,7 @@ __global__ __launch_bounds__(256) void k_reduce(DevState d,
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double* out = d.red + ((size_t)s * kRedChunks + chunk) * kRedFields;
+        double* out = d.red + (size_t)slot * kRedFields;
         for (int f = 0; f < kRedFields; ++f) out[f] = sh[f][0];
     }
 }
@@ -1192,19 +886,41 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
 // ---------------------------------------------------------------------------
 // End of step: per-scene dt, conservation sums and counters; global counts.
 // ---------------------------------------------------------------------------
-__global__ void k_finalize(DevState d) {
-    GRID_STRIDE(s, d.S) {
-        SceneDev& sc = d.scenes[s];
-        SceneCount& c = d.scnt[s];
-        if (!sc.alive) continue;
-        double tot[kRedFields] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int ch = 0; ch < kRedChunks; ++ch) {
-            const double* r = d.red + ((size_t)s * kRedChunks + ch) * kRedFields;
+// One block per scene: the scene's chunk partials are summed in a fixed
+// order (thread t takes slots t, t+256, ...; then a fixed tree), so the
+// sums are bitwise reproducible.
+__global__ __launch_bounds__(256) void k_finalize(DevState d) {
+    __shared__ double sh[kRedFields][256];
+    const int s = blockIdx.x;
+    SceneDev& sc = d.scenes[s];
+    SceneCount& c = d.scnt[s];
+    const int nslots = d.red_n[s];
+    if (sc.alive) {
+        double part[kRedFields] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ch = threadIdx.x; ch < nslots; ch += 256) {
+            const double* r = d.red + ((size_t)d.red_base[s] + ch) * kRedFields;
             for (int f = 0; f < kRedFields; ++f) {
-                if (f == 5) tot[f] = fmax(tot[f], r[f]);
-                else tot[f] += r[f];
+                if (f == 5) part[f] = fmax(part[f], r[f]);
+                else part[f] += r[f];
             }
         }
+        for (int f = 0; f < kRedFields; ++f) sh[f][threadIdx.x] = part[f];
+    }
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (sc.alive && threadIdx.x < (unsigned)off) {
+            for (int f = 0; f < kRedFields; ++f) {
+                if (f == 5)
+                    sh[f][threadIdx.x] = fmax(sh[f][threadIdx.x], sh[f][threadIdx.x + off]);
+                else
+                    sh[f][threadIdx.x] += sh[f][threadIdx.x + off];
+            }
+        }
+        __syncthreads();
+    }
+    if (sc.alive && threadIdx.x == 0) {
+        double tot[kRedFields];
+        for (int f = 0; f < kRedFields; ++f) tot[f] = sh[f][0];
         c.n_l_last = c.seg_l1 - c.seg_l0;
         c.n_h_last = c.seg_h1 - c.seg_h0;
         // next lists: low = kept + merged; high = kept + 4 per split
@@ -1214,10 +930,10 @@ __global__ void k_finalize(DevState d) {
         c.n_g = c.n_g + c.n_split - c.n_merge - c.n_dissolve;
         c.step += 1;
         // rounding bound of the sums: per-term rounding (<= 2u) + the
-        // serial per-thread loop + 8-level tree + kRedChunks serial sum
-        const int m = c.n_l_last + c.n_h_last;
-        const double serial = (double)(m / (kRedChunks * 256) + 1);
-        const double kk = serial + 8.0 + (double)kRedChunks + 2.0;
+        // serial per-thread loop + 8-level tree of k_reduce, then the
+        // per-thread slot loop + 8-level tree here
+        const double serial = (double)(kRedChunk / 256);
+        const double kk = serial + 8.0 + (double)((nslots + 255) / 256) + 8.0 + 2.0;
         const double u = 1.1102230246251565e-16;
         const double gam = kk * u / (1.0 - kk * u);
         double* o = d.cons + (size_t)s * 10;
@@ -1337,13 +1053,14 @@ __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
         const bool high = k >= nl;
         const bool i_prom = !(si & 1u);
         w[7] += 1;
-        // same level
+        // same level (rows skipped exactly as the neighbour kernels do)
         {
             Box b = scan_box(sc, high, xi, high ? sc.H : sc.h);
+            const uint8_t need = (high && i_prom) ? 1u : 0u;
             for (int cy = b.y0; cy <= b.y1; ++cy) {
                 int st, en;
                 int row = high ? sc.cbase_h + cy * sc.dhx : sc.cbase_l + cy * sc.dlx;
-                row_run(d.cstart, d.cend, row, b.x0, b.x1, st, en);
+                row_run_need(d.cstart, d.cend, d.cflag, row, b.x0, b.x1, need, st, en);
                 for (int j = st; j < en; ++j) {
                     float r2 = dist2(xi, d.spos[j]);
                     const uint8_t sj = d.sstate[j];
@@ -1357,10 +1074,11 @@ __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
         // cross level
         if (high || d.scnt[s].seg_h1 > d.scnt[s].seg_h0) {
             Box b = scan_box(sc, !high, xi, sc.h);
+            const uint8_t need = high ? (i_prom ? 0u : 1u) : 1u;
             for (int cy = b.y0; cy <= b.y1; ++cy) {
                 int st, en;
                 int row = !high ? sc.cbase_h + cy * sc.dhx : sc.cbase_l + cy * sc.dlx;
-                row_run(d.cstart, d.cend, row, b.x0, b.x1, st, en);
+                row_run_need(d.cstart, d.cend, d.cflag, row, b.x0, b.x1, need, st, en);
                 for (int j = st; j < en; ++j) {
                     float r2 = dist2(xi, d.spos[j]);
                     const uint8_t sj = d.sstate[j];
@@ -1488,11 +1206,15 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     const int cur = enqueue_grid(d, sh, st);
     const uint32_t* vals = d.vals[cur];
     if (ev) (void)hipEventRecord(ev[1], st);
-    k_density<<<sh.grid_elem, sh.dens_block, 0, st>>>(d, vals);
+    const int g_low = blocks_for(d.CL, kPairBlock, 1 << 30);
+    const int g_high = blocks_for(4LL * d.CL, kPairBlock, 1 << 30);
+    k_density<false><<<g_low, kPairBlock, 0, st>>>(d, vals);
+    k_density<true><<<g_high, kPairBlock, 0, st>>>(d, vals);
     if (ev) (void)hipEventRecord(ev[2], st);
-    k_force<<<sh.grid_elem, sh.force_block, 0, st>>>(d, vals);
+    k_force<false><<<g_low, kPairBlock, 0, st>>>(d, vals);
+    k_force<true><<<g_high, kPairBlock, 0, st>>>(d, vals);
     if (ev) (void)hipEventRecord(ev[3], st);
-    k_reduce<<<dim3(kRedChunks, sh.S), 256, 0, st>>>(d, vals);
+    k_reduce<<<sh.red_slots, 256, 0, st>>>(d, vals);
     k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
     k_groups<<<blocks_for(d.CL, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
     k_cap<<<sh.S, 256, 0, st>>>(d);
@@ -1504,7 +1226,7 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
         d, sh, vals, d.list_g[par], d.list_l[par ^ 1], d.list_h[par ^ 1],
         d.list_g[par ^ 1]);
     if (ev) (void)hipEventRecord(ev[4], st);
-    k_finalize<<<blocks_for(sh.S, 64, 1024), 64, 0, st>>>(d);
+    k_finalize<<<sh.S, 256, 0, st>>>(d);
     if (ev) (void)hipEventRecord(ev[5], st);
 }
 
