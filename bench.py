@@ -90,6 +90,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--natural", action="store_true",
                     help="surface-band C3 instead of the forced all-split")
+    ap.add_argument("--copy-probe-mb", type=int, default=512,
+                    help="plain-copy bandwidth probe size (0: skip); also "
+                         "the reference dispatch for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
@@ -206,6 +209,13 @@ def main() -> int:
         },
         "work": work,
     }
+    if args.copy_probe_mb > 0:
+        import ctypes
+        from twoscalesplit_amd import _native
+        g = ctypes.c_double()
+        _native.check(_native.lib().tss_bandwidth_probe(
+            args.copy_probe_mb << 20, 3, ctypes.byref(g)), "bandwidth probe")
+        out["copy_probe_gbs"] = round(g.value, 1)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         _log("cpu baseline", args.cpu_steps, "steps,", threads, "threads")

@@ -254,6 +254,12 @@ int tss_scene_dam_break(uint64_t seed, uint64_t scene, int nx, int ny,
 /* SplitMix64 output stream (for tests of the generator). */
 uint64_t tss_splitmix64_next(uint64_t* state);
 
+/* Streaming-copy bandwidth of the device (read + write bytes / time) over
+ * `bytes`-sized buffers, `reps` copies: the plain-copy reference of the
+ * roofline and of the HBM counter calibration. No counterpart in the
+ * reference. */
+int tss_bandwidth_probe(size_t bytes, int reps, double* gbs);
+
 const char* tss_last_error(void);
 const char* tss_version(void);
 

@@ -123,9 +123,8 @@ struct DevState {
     float2* bpos;             // [NB] boundary positions in cell order
     int* bids;                // [NB] boundary ids (scene-local) in cell order
     float2* spos;
-    float2* svel;             // [5 CL] cell-sorted copies
-    float* srho;
-    float* spr;               // p / rho^2
+    float4* sxv;              // [5 CL] cell-sorted (x, y, vx, vy)
+    float2* srp;              // [5 CL] cell-sorted (rho, p / rho^2)
     uint8_t* sstate;
     uint8_t* ssurf;           // surface flag (sorted)
     int* sscene;
@@ -166,6 +165,7 @@ void launch_grid_only(const DevState& d, const StepShape& sh, void* stream);
 void launch_work_counts(const DevState& d, const StepShape& sh,
                         unsigned long long* out, void* stream);
 void launch_fill_int(int* p, int value, long long n, void* stream);
+void launch_copy16(const void* src, void* dst, long long n16, void* stream);
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
                          void* stream);
 

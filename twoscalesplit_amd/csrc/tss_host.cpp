@@ -226,7 +226,7 @@ struct tss_batch {
         b += 256 * tiles * 4 + tss::kMaxPasses * 256 * 4;
         b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 8;
         b += ds.cap_bnd * 12;
-        b += U * (8 + 8 + 4 + 4 + 1 + 1 + 4 + 4);  // sorted copies + sidx
+        b += U * (8 + 16 + 8 + 1 + 1 + 4 + 4);  // sorted copies + sidx
         b += CL * 2;
         b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) + 10 * 8 + 12);
         b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
@@ -282,9 +282,8 @@ struct tss_batch {
         d.bpos = dalloc<float2>(cap_b, owned);
         d.bids = dalloc<int>(cap_b, owned);
         d.spos = dalloc<float2>(U, owned);
-        d.svel = dalloc<float2>(U, owned);
-        d.srho = dalloc<float>(U, owned);
-        d.spr = dalloc<float>(U, owned);
+        d.sxv = dalloc<float4>(U, owned);
+        d.srp = dalloc<float2>(U, owned);
         d.sstate = dalloc<uint8_t>(U, owned);
         d.ssurf = dalloc<uint8_t>(U, owned);
         d.sscene = dalloc<int>(U, owned);
@@ -1285,6 +1284,43 @@ int tss_batch_set_tuning(tss_batch* b, int dens_block, int force_block) {
 }
 
 void tss_batch_destroy(tss_batch* b) { delete b; }
+
+int tss_bandwidth_probe(size_t bytes, int reps, double* gbs) {
+    return guarded([&] {
+        require(gbs && bytes >= 16 && reps >= 1, "invalid argument");
+        const size_t n16 = bytes / 16;
+        void *a = nullptr, *b = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+        };
+        try {
+            HIP_CHECK(hipMalloc(&a, n16 * 16));
+            HIP_CHECK(hipMalloc(&b, n16 * 16));
+            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            HIP_CHECK(hipMemsetAsync(a, 1, n16 * 16, st));
+            tss::launch_copy16(a, b, (long long)n16, st);  // warm
+            HIP_CHECK(hipEventRecord(e0, st));
+            for (int i = 0; i < reps; ++i) tss::launch_copy16(a, b, (long long)n16, st);
+            HIP_CHECK(hipEventRecord(e1, st));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            *gbs = 2.0 * (double)(n16 * 16) * reps / (ms * 1e-3) / 1e9;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
 
 const char* tss_last_error(void) { return g_last_error.c_str(); }
 

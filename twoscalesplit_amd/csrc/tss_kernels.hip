@@ -381,8 +381,10 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
         // states present per cell (every particle stores the same 1 for its
         // state): lets the scans skip rows whose weights are all zero
         d.cflag[2 * (size_t)key + (d.state[u] & 1u)] = 1;
-        d.spos[k] = d.pos[u];
-        d.svel[k] = d.vel[u];
+        const float2 x = d.pos[u];
+        const float2 v = d.vel[u];
+        d.spos[k] = x;
+        d.sxv[k] = make_float4(x.x, x.y, v.x, v.y);
         d.sstate[k] = d.state[u];
         d.sidx[u] = k;
         const int s = d.scene_of[low_of(d, u)];
@@ -1161,6 +1163,20 @@ static inline int blocks_for(long long n, int bs, int cap) {
     if (b < 1) b = 1;
     if (b > cap) b = cap;
     return (int)b;
+}
+
+// Plain streaming copy (16-byte vectors): the reference workload of the
+// bandwidth probe and of the HBM counter calibration.
+__global__ void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst,
+                         long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy16(const void* src, void* dst, long long n16, void* stream) {
+    k_copy16<<<blocks_for(n16, 256, 8192), 256, 0, (hipStream_t)stream>>>(
+        (const float4*)src, (float4*)dst, n16);
 }
 
 void launch_fill_int(int* p, int value, long long n, void* stream) {
