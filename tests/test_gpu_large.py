@@ -1,0 +1,87 @@
+"""Full-size invariants of the large forced two-scale dam break
+(1,414 x 707 = 999,698 low particles, every one split at the first step into
+3,998,792 high children), where the oracle is too slow to run beside it:
+
+* the lists are duplicate-free sets of valid ids; n_high = 4 n_groups and the
+  listed children are exactly the children of the listed groups;
+* keys are sorted, the sorted ids are a permutation of the advanced list,
+  and the cell ranges partition the sorted arrays, each cell's keys equal
+  to the cell index;
+* mass is exact (counts times masses) and the FP64 conservation sums agree
+  with a host summation of the read-back state within their error bound;
+* every transient child lies within the clamp radius
+  0.75 sqrt(m / (rho pi)) of its parent; every value is finite.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from twoscalesplit_amd import Solver, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_grid(sol, cnt, level):
+    L = "L" if level == 0 else "H"
+    keys = sol.read("KEYS_" + L, cnt)
+    ids = sol.read("SORTED_" + L, cnt)
+    st = sol.read("CELL_START_" + L, cnt)
+    en = sol.read("CELL_END_" + L, cnt)
+    assert np.all(np.diff(keys) >= 0)
+    occ = st >= 0
+    assert np.array_equal(occ, en >= 0)
+    lens = en[occ] - st[occ]
+    assert np.all(lens > 0) and lens.sum() == len(keys)
+    cells = np.nonzero(occ)[0]
+    # each occupied cell's range holds exactly its keys, in order
+    assert np.array_equal(np.repeat(cells, lens), keys)
+    assert np.array_equal(st[occ][1:], en[occ][:-1])
+    return ids
+
+
+def test_large_forced_split_invariants():
+    sc = scenes.c3_large(forced=True)
+    cfg = sc.cfg
+    print("[case] large forced split", sc.n_low, flush=True)
+    sol = Solver(cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    try:
+        sol.step(3)
+        cnt = sol.counts()
+        nl, nh, ng = cnt["n_low"], cnt["n_high"], cnt["n_groups"]
+        print("[case] counts", cnt, flush=True)
+        assert ng == sc.n_low and nh == 4 * ng and nl == sc.n_low
+        ll, lh, lg = (sol.read(f, cnt) for f in ("LIST_L", "LIST_H", "LIST_G"))
+        assert len(np.unique(ll)) == nl and ll.min() >= 0 and ll.max() < sc.cap_low
+        assert len(np.unique(lg)) == ng
+        assert np.array_equal(np.sort(lh), np.sort((4 * lg[:, None] + np.arange(4)).ravel()))
+        # grid of the last step (it advanced the same lists)
+        ids_l = _check_grid(sol, cnt, 0)
+        ids_h = _check_grid(sol, cnt, 1)
+        assert np.array_equal(np.sort(ids_l), np.sort(ll))
+        assert np.array_equal(np.sort(ids_h), np.sort(lh))
+        # state
+        pl, ph = sol.read("POS_L", cnt), sol.read("POS_H", cnt)
+        vl, vh = sol.read("VEL_L", cnt), sol.read("VEL_H", cnt)
+        rl = sol.read("RHO_L", cnt)
+        sl, sh = sol.read("STATE_L", cnt), sol.read("STATE_H", cnt)
+        for arr in (pl[ll], ph[lh], vl[ll], vh[lh], rl[ll]):
+            assert np.all(np.isfinite(arr))
+        assert np.all(sl[lg] == 1) and np.all(sh[lh] == 1)
+        # clamp radius (transient children stay near their parent)
+        m = float(cfg.mass_low)
+        par = lh // 4                       # child 4p + k belongs to parent p
+        rmax = 0.75 * np.sqrt(m / (rl[par].astype(np.float64) * math.pi))
+        d = np.linalg.norm(ph[lh].astype(np.float64) - pl[par].astype(np.float64),
+                           axis=1)
+        assert np.all(d <= rmax * (1 + 1e-5)), (d / rmax).max()
+        # mass and FP64 sums (transient children carry no mass of their own)
+        out, err = sol.conservation()
+        assert out[0] == nl * m
+        px = math.fsum((m * vl[ll, 0].astype(np.float64)).tolist())
+        tol = 1e-12 * math.fsum(np.abs(m * vl[ll, 0].astype(np.float64)).tolist()) + err[1]
+        assert abs(out[1] - px) <= tol, (out[1], px, tol)
+    finally:
+        sol.close()

@@ -1,0 +1,375 @@
+"""Pins of the oracle (CPU restatement) against closed forms, brute force and
+invariants -- chosen so that a plausible mistake (dropped term, wrong sign,
+transposed operand, off-by-one cell) fails one of them. CPU only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import binding as ob
+
+L = ob.lib()
+
+
+# ---------------------------------------------------------------- kernels --
+@pytest.mark.parametrize("ell", [0.025, 0.0125, 4.0 * 0.8 / 707])
+def test_m4_normalisation_and_value_at_zero(ell):
+    # 2-D normalisation: int W(r) 2 pi r dr = 1 (R/WCSPHSolver.cu:2559-2590)
+    val, _ = integrate.quad(lambda r: L.orc_kernel_w(r, ell) * 2 * math.pi * r,
+                            0.0, ell, points=[ell / 2], epsabs=1e-14,
+                            epsrel=1e-12)
+    assert abs(val - 1.0) < 1e-10
+    assert L.orc_kernel_w(0.0, ell) == pytest.approx(40.0 / (7 * math.pi * ell * ell),
+                                                    rel=1e-14)
+    assert L.orc_kernel_w(ell, ell) == 0.0
+    assert L.orc_kernel_w(1.5 * ell, ell) == 0.0
+
+
+@pytest.mark.parametrize("ell", [0.025, 0.0125])
+def test_m4_continuity_and_gradient(ell):
+    eps = 1e-12 * ell
+    for r in (ell / 2, ell):  # q = 1 and q = 2
+        a = L.orc_kernel_w(r - eps, ell)
+        b = L.orc_kernel_w(r + eps, ell)
+        assert abs(a - b) <= 1e-9 * L.orc_kernel_w(0.0, ell)
+    # dW/dr equals the derivative of W (central differences)
+    for r in np.linspace(0.02, 0.98, 37) * ell:
+        hstep = 1e-6 * ell
+        fd = (L.orc_kernel_w(r + hstep, ell) - L.orc_kernel_w(r - hstep, ell)) / (2 * hstep)
+        an = L.orc_kernel_gradf(r, ell)
+        assert an == pytest.approx(fd, rel=1e-6, abs=1e-9 * abs(fd) + 1e-6)
+    # the gradient vanishes at r -> 0 and beyond the support
+    assert abs(L.orc_kernel_gradf(1e-9 * ell, ell)) < 1e-3 * abs(
+        L.orc_kernel_gradf(0.4 * ell, ell))
+    assert L.orc_kernel_gradf(1.01 * ell, ell) == 0.0
+
+
+def test_wall_weight_closed_form():
+    # Gamma = 0.02 c^2 / r^2 * {2/3, 2q - 1.5 q^2, 0.5 (2 - q)^2, 0}; the
+    # constant is the single-precision literal 0.02f (R/WCSPHSolver.cu:70)
+    ell, c = 0.025, 39.6
+    k002 = float(np.float32(0.02))
+    for q, shape in ((0.3, 2 / 3), (0.8, 2 * 0.8 - 1.5 * 0.64),
+                     (1.5, 0.5 * 0.25), (2.5, 0.0)):
+        r = q * ell / 2
+        want = k002 * c * c / (r * r) * shape
+        assert L.orc_wall_gamma(r, ell, c) == pytest.approx(want, rel=1e-12)
+    for q in (2 / 3, 1.0, 2.0):  # continuous at the breakpoints
+        r = q * ell / 2
+        a = L.orc_wall_gamma(r * (1 - 1e-12), ell, c)
+        b = L.orc_wall_gamma(r * (1 + 1e-12), ell, c)
+        assert a == pytest.approx(b, rel=1e-8, abs=1e-6)
+
+
+def test_tait():
+    rho0, B = 1000.0, 224228.6
+    assert L.orc_tait(rho0, rho0, B) == 0.0
+    d = 1e-4 * rho0
+    slope = (L.orc_tait(rho0 + d, rho0, B) - L.orc_tait(rho0 - d, rho0, B)) / (2 * d)
+    assert slope == pytest.approx(7 * B / rho0, rel=1e-6)  # = c0^2
+    assert L.orc_tait(1.1 * rho0, rho0, B) == pytest.approx(B * (1.1 ** 7 - 1), rel=1e-13)
+
+
+# ------------------------------------------------------------------- grid --
+def _axis_ref(x, off, o, cell, dim):
+    t = np.float32(np.float32(np.float32(x) + np.float32(off)) - np.float32(o)) / np.float32(cell)
+    t = np.float32(t)
+    if not (t >= 1.0):
+        return 0
+    if t >= np.float32(dim):
+        return dim - 1
+    return int(t)
+
+
+def test_axis_cell_matches_fp32_definition():
+    rng = np.random.default_rng(1)
+    xs = list(rng.uniform(-0.1, 1.1, 2000).astype(np.float32))
+    xs += [0.0, -0.0, 0.025, 0.05, 1.0, 0.999999, np.inf, -np.inf, np.nan, 1e30]
+    for x in xs:
+        for off in (0.0, 0.025, -0.025):
+            with np.errstate(invalid="ignore", over="ignore"):
+                want = _axis_ref(x, off, 0.0, 0.025, 40)
+            got = L.orc_axis_cell(float(x), off, 0.0, 0.025, 40)
+            assert got == want, (x, off, got, want)
+
+
+def test_lattice_hash_is_integer_cell():
+    p = ob.OrcParams()
+    assert L.orc_params_init(C.byref(p), 0, 0, 1, 1, 0.025, 0.0125, 1000, 1,
+                             1, 0, 0) == 0
+    assert list(p.dims_l) == [40, 40] and list(p.dims_h) == [80, 80]
+    for i in range(40):
+        for j in (0, 7, 39):
+            x = np.float32((i + 0.5) * 0.025)
+            y = np.float32((j + 0.5) * 0.025)
+            assert L.orc_hash(C.byref(p), 0, x, y) == j * 40 + i
+    assert L.orc_params_init(C.byref(p), 1, 0, 1, 1, 0.025, 0.0125, 1000, 1,
+                             1, 0, 0) == -1  # empty domain is rejected
+
+
+def test_stable_sort_and_cell_ranges():
+    rng = np.random.default_rng(2)
+    n, nk = 5000, 300
+    keys = rng.integers(0, nk, n).astype(np.int32)
+    vals = rng.permutation(n).astype(np.int32)
+    ko = np.zeros(n, np.int32)
+    vo = np.zeros(n, np.int32)
+    L.orc_stable_sort(ob.ptr(keys), ob.ptr(vals), n, nk, ob.ptr(ko), ob.ptr(vo))
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(ko, keys[order])
+    np.testing.assert_array_equal(vo, vals[order])
+    st = np.zeros(nk + 5, np.int32)
+    en = np.zeros(nk + 5, np.int32)
+    L.orc_cell_ranges(ob.ptr(ko), n, nk + 5, ob.ptr(st), ob.ptr(en))
+    for c in range(nk + 5):
+        idx = np.nonzero(ko == c)[0]
+        if len(idx) == 0:
+            assert st[c] == -1 and en[c] == -1
+        else:
+            assert st[c] == idx[0] and en[c] == idx[-1] + 1
+            assert np.all(np.diff(idx) == 1)
+
+
+# ------------------------------------------------------------ neighbours --
+def _tiny_two_level(seed=3):
+    """Tiny state with both levels (the oracle's own split step)."""
+    from twoscalesplit_amd import MODE_REFERENCE, scenes
+    sc = scenes.small_two_scale(nx=10, ny=12, mode=MODE_REFERENCE, split_x=0.03,
+                                nxd=24, nyd=20)
+    o = ob.Scene.from_initial(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    o.step(64, 1)
+    o.step(64, 1)
+    assert o.s.n_h > 0
+    return sc, o
+
+
+def _dist32(a, b):
+    dx = np.float32(a[0]) - np.float32(b[0])
+    dy = np.float32(a[1]) - np.float32(b[1])
+    return np.sqrt(np.float32(np.float32(dx * dx) + np.float32(dy * dy)))
+
+
+def test_neighbour_sets_equal_brute_force():
+    sc, o = _tiny_two_level()
+    g = o.grid()
+    h, H = np.float32(sc.cfg.effective_radius), np.float32(sc.cfg.effective_radius_high)
+    nl, nh, _ = o.n
+    ll, lh = o.a["list_l"][:nl], o.a["list_h"][:nh]
+    PL, PH = o.a["pos_l"], o.a["pos_h"]
+    csum = lambda ids: int(np.sum((np.asarray(ids, np.uint64) + 1) * 2654435761) % (1 << 32))
+    for level, lst, P, Q, lq in ((0, ll, PL, PH, lh), (1, lh, PH, PL, ll)):
+        counts, sums = o.neighbors(g, level)
+        same_list = lst
+        rd = H if level else h
+        for t, i in enumerate(lst):
+            ds = np.array([_dist32(P[i], P[j]) for j in same_list])
+            dc = np.array([_dist32(P[i], Q[j]) for j in lq]) if len(lq) else np.zeros(0)
+            sets = [same_list[ds <= rd], lq[dc <= (H if level else h)],
+                    same_list[(ds != 0) & (ds < rd)], lq[(dc != 0) & (dc < h)]]
+            for q in range(4):
+                assert counts[t, q] == len(sets[q]), (level, i, q)
+                assert sums[t, q] == csum(sets[q]), (level, i, q)
+
+
+# ---------------------------------------------------------------- physics --
+def _random_state(two_level: bool, seed=4):
+    from twoscalesplit_amd import MODE_PLAIN, MODE_REFERENCE, scenes
+    sc = scenes.small_two_scale(nx=12, ny=12, nxd=40, nyd=40,
+                                mode=MODE_REFERENCE if two_level else MODE_PLAIN,
+                                split_x=-1.0)
+    cfg = sc.cfg
+    cfg.gravity = 0.0
+    # tension is a per-unit-mass acceleration (-sigma sum W x_ij,
+    # R/WCSPHSolver.cu:1790-1805): antisymmetric only between equal masses
+    cfg.tension_coefficient = 0.0 if two_level else 0.3
+    o = ob.Scene(ob.params_from_config(cfg), sc.cap_low, np.zeros((0, 2), np.float32))
+    rng = np.random.default_rng(seed)
+    n = sc.n_low
+    dx = sc.meta["dx"]
+    o.a["pos_l"][:n] = sc.low_xy + np.float32(10 * dx)
+    o.a["vel_l"][:n] = rng.normal(0, 0.5, (n, 2)).astype(np.float32)
+    o.a["list_l"][:n] = np.arange(n)
+    o.s.n_l = n
+    if two_level:
+        # every low particle active; its four children promoted (state 0)
+        k = np.arange(4 * n)
+        o.a["pos_h"][k] = (np.repeat(o.a["pos_l"][:n], 4, axis=0) +
+                           rng.uniform(-0.3 * dx, 0.3 * dx, (4 * n, 2))).astype(np.float32)
+        o.a["vel_h"][k] = rng.normal(0, 0.5, (4 * n, 2)).astype(np.float32)
+        o.a["state_h"][k] = 0
+        o.a["list_h"][:4 * n] = k
+        o.s.n_h = 4 * n
+    g = o.grid()
+    rl, pl, rh, ph = o.density64(g)
+    o.a["rho_l"][:] = rl
+    o.a["p_l"][:] = pl
+    o.a["rho_h"][:] = rh
+    o.a["p_h"][:] = ph
+    return cfg, o, g
+
+
+@pytest.mark.parametrize("two_level", [False, True])
+def test_pair_forces_conserve_momentum(two_level):
+    """Pressure and viscosity pair terms are antisymmetric, across levels too
+    (lambda_LH[0,0] = lambda_HL[0,0] = 1, same cutoff and viscosity length
+    both ways), and so is tension within one level: with g = 0 and no
+    walls, sum m a = 0."""
+    cfg, o, g = _random_state(two_level)
+    al, ah, sl, sh = o.force64(g)
+    nl, nh, _ = o.n
+    ll, lh = o.a["list_l"][:nl], o.a["list_h"][:nh]
+    tot = cfg.mass_low * al[ll].sum(axis=0)
+    scale = cfg.mass_low * sl[ll].sum()
+    if two_level:
+        tot = tot + cfg.mass_high * ah[lh].sum(axis=0)
+        scale += cfg.mass_high * sh[lh].sum()
+    assert np.all(np.abs(tot) <= 1e-12 * scale), (tot, scale)
+
+
+def test_interior_lattice_density_near_rest():
+    cfg, o, g = _random_state(False)
+    nl = o.s.n_l
+    # interior particles of the jittered lattice (two rows away from edges)
+    P = o.a["pos_l"][:nl]
+    c = P.mean(axis=0)
+    inner = np.all(np.abs(P - c) < 0.25 * (P.max(axis=0) - P.min(axis=0)), axis=1)
+    rho = o.a["rho_l"][:nl][inner]
+    assert np.all(np.abs(rho / cfg.rest_density - 1) < 0.02), rho
+
+
+def test_fp32_transcription_agrees_with_fp64():
+    cfg, o, g = _random_state(True)
+    cl = o.cap_l
+    out32 = [np.zeros(cl, np.float32), np.zeros(cl, np.float32),
+             np.zeros(4 * cl, np.float32), np.zeros(4 * cl, np.float32)]
+    L.orc_density_f32(C.byref(o.p), C.byref(o.s), C.byref(g["_struct"]),
+                      *[ob.ptr(a) for a in out32])
+    rl, pl, rh, ph = o.density64(g)
+    nl, nh, _ = o.n
+    ll, lh = o.a["list_l"][:nl], o.a["list_h"][:nh]
+    assert np.max(np.abs(out32[0][ll] / rl[ll] - 1)) < 1e-5
+    assert np.max(np.abs(out32[2][lh] / rh[lh] - 1)) < 1e-5
+
+
+# -------------------------------------------------------------- adaptivity --
+def test_split_geometry_and_clamp():
+    from twoscalesplit_amd import MODE_REFERENCE, scenes
+    sc = scenes.small_two_scale(nx=8, ny=8, mode=MODE_REFERENCE, split_x=-1.0)
+    o = ob.Scene.from_initial(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    o.step(64, 1)
+    nl, nh, ng = o.n
+    assert ng == nl == 64 and nh == 4 * 64
+    m = sc.cfg.mass_low
+    assert 4 * np.float32(sc.cfg.mass_high) == np.float32(m)
+    for p in o.a["list_g"][:ng]:
+        ch = o.a["pos_h"][4 * p:4 * p + 4].astype(np.float64)
+        par = o.a["pos_l"][p].astype(np.float64)
+        rho = float(o.a["rho_l"][p])
+        rc = 0.5 * math.sqrt(m / (2 * rho * math.pi))
+        np.testing.assert_allclose(ch.mean(axis=0), par, atol=4e-7)
+        np.testing.assert_allclose(np.abs(ch - par), rc, rtol=1e-3)
+        # order (+,+), (-,+), (+,-), (-,-)
+        assert ch[0, 0] > par[0] and ch[1, 0] < par[0] and ch[2, 1] < par[1]
+        assert np.all(o.a["state_h"][4 * p:4 * p + 4] == 1)
+    # after more steps every transient child stays within the clamp radius
+    for _ in range(5):
+        o.step(64, 1)
+    for p in o.a["list_g"][:o.s.n_g]:
+        ch = o.a["pos_h"][4 * p:4 * p + 4].astype(np.float64)
+        par = o.a["pos_l"][p].astype(np.float64)
+        rmax = 0.75 * math.sqrt(m / (float(o.a["rho_l"][p]) * math.pi))
+        assert np.all(np.linalg.norm(ch - par, axis=1) <= rmax * (1 + 1e-6))
+
+
+def test_lists_stay_consistent_under_band_rules():
+    from twoscalesplit_amd import MODE_BAND, scenes
+    sc = scenes.small_two_scale(nx=16, ny=20, mode=MODE_BAND)
+    sc.cfg.promote_min_age = 1
+    sc.cfg.promote_max_age = 2
+    o = ob.Scene.from_initial(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    seen_split = seen_prom = 0
+    for _ in range(12):
+        o.step(64, 2)
+        ll, lh, lg = o.lists()
+        assert len(set(ll)) == len(ll) and len(set(lh)) == len(lh)
+        # children listed = exactly the children of the listed groups
+        assert sorted(lh) == sorted((4 * lg[:, None] + np.arange(4)).ravel())
+        st = o.a["state_l"]
+        assert np.all(st[ll] != 2)                      # deleted never listed
+        # every fluid particle is represented exactly once: listed low, or a
+        # deleted (promoted) parent whose children carry its mass
+        promoted = set(lg[st[lg] == 2].tolist())
+        assert set(ll.tolist()).isdisjoint(promoted)
+        assert set(ll.tolist()) | promoted == set(range(sc.n_low))
+        # mass: listed low + promoted children = initial fluid mass
+        real_h = lh[o.a["state_h"][lh] == 0]
+        mass = len(ll) * sc.cfg.mass_low + len(real_h) * sc.cfg.mass_high
+        assert mass == pytest.approx(sc.n_low * sc.cfg.mass_low, rel=1e-6)
+        seen_split += int(np.sum(st[lg] == 1))
+        seen_prom += int(np.sum(st[lg] == 2))
+    assert seen_split > 0 and seen_prom > 0
+
+
+def test_band_test_against_brute_force():
+    from twoscalesplit_amd import MODE_BAND, scenes
+    sc = scenes.small_two_scale(nx=16, ny=20, mode=MODE_BAND)
+    o = ob.Scene.from_initial(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    g = o.grid()
+    rl, pl, rh, ph = o.density64(g)
+    o.a["rho_l"][:] = rl
+    o.a["p_l"][:] = pl
+    pre_l, pre_h = o.a["pos_l"].copy(), o.a["pos_h"].copy()
+    before = o.a["state_l"].copy()
+    out = o.adapt(g, pre_l, pre_h)
+    n = sc.n_low
+    P = pre_l[:n]
+    h = np.float32(sc.cfg.effective_radius)
+    B = sc.bnd_xy
+    surf = np.zeros(n, bool)
+    for i in range(n):
+        if o.a["rho_l"][i] < np.float32(np.float32(sc.cfg.surface_ratio) * np.float32(1000.0)):
+            dw = [_dist32(P[i], b) for b in B]
+            surf[i] = not np.any(np.array(dw) <= h)
+    np.testing.assert_array_equal(out["surf_l"][:n].astype(bool), surf)
+    want = []
+    for i in g["sorted_l"]:
+        if before[i] != 0:
+            continue
+        d = np.array([_dist32(P[i], P[j]) for j in np.nonzero(surf)[0]])
+        if len(d) and np.any(d <= h):
+            want.append(i)
+    np.testing.assert_array_equal(out["split_ids"], np.array(want, np.int32))
+
+
+def test_cfl_dt_closed_form():
+    from twoscalesplit_amd import DT_CFL, MODE_PLAIN, scenes
+    sc = scenes.small_two_scale(nx=8, ny=8, mode=MODE_PLAIN, dt_cfl=True)
+    o = ob.Scene.from_initial(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    o.a["vel_l"][:64] = 0
+    o.a["vel_l"][17] = (3.0, -4.0)
+    v2 = o.v2max()
+    assert v2 == 25.0
+    dt = o.next_dt(v2)
+    want = min(sc.cfg.dt_max, sc.cfg.cfl * 0.5 * sc.cfg.effective_radius / (sc.cfg.speed_sound + 5.0))
+    assert dt == np.float32(want)
+
+
+def test_conservation_sums():
+    from twoscalesplit_amd import MODE_BAND, scenes
+    sc = scenes.small_two_scale(nx=16, ny=20, mode=MODE_BAND)
+    o = ob.Scene.from_initial(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    for _ in range(3):
+        o.step(64, 1)
+    out, ab = o.conservation()
+    ll, lh, _ = o.lists()
+    real = lh[o.a["state_h"][lh] == 0]
+    m_l, m_h = float(sc.cfg.mass_low), float(sc.cfg.mass_high)
+    assert out[0] == pytest.approx(len(ll) * m_l + len(real) * m_h, rel=1e-15)
+    V = o.a["vel_l"][ll].astype(np.float64)
+    px = math.fsum(list(m_l * V[:, 0]) + list(m_h * o.a["vel_h"][real, 0].astype(np.float64)))
+    assert out[1] == pytest.approx(px, rel=1e-14, abs=1e-18)
