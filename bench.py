@@ -61,6 +61,22 @@ def grid_bytes(n: int, cells: int, passes: int) -> float:
     return n * (16 + 16 * passes + 4 + 2 * S) + 16 * cells
 
 
+def _traffic(kernel: str) -> dict | None:
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, each scaled by the
+    reading of a plain copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "pmc_c3_forced.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 6 and parts[0] == "traffic" and parts[1] == kernel:
+                return {"bytes": int(parts[2]),
+                        "source": "profiles/pmc_c3_forced.txt"}
+    return None
+
+
 def cpu_baseline(snap: dict, sc, steps: int, threads: int) -> dict:
     """The oracle's FP32 transcription of the reference kernels (OpenMP)
     on the GPU state of the same workload."""
@@ -86,7 +102,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--latency-steps", type=int, default=60)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--natural", action="store_true",
                     help="surface-band C3 instead of the forced all-split")
@@ -104,23 +120,41 @@ def main() -> int:
     _log("scene")
     sc = scenes.c3_large(forced=not args.natural)
     sol = Solver(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    # warmup builds both graph variants (plain and timed)
     _log("warmup", args.warmup)
     sol.step_graph(args.warmup)
+    sol.enable_timing(True)
+    sol.step_graph(max(4, args.warmup // 4))
     sol.synchronize()
+    w0 = sol.work_counts()
     c0 = sol.counts()
     n0 = c0["n_low"] + c0["n_high"]
+
+    # timed region: K graph-replayed steps; hipEvents inside the graphs
+    # time every phase of every step on the solver's stream
+    sol.enable_timing(True)
     _log("timed", args.steps, "steps of", n0, "particles")
     t0 = time.perf_counter()
     sol.step_graph(args.steps)
     sol.synchronize()
     t1 = time.perf_counter()
+    ph = sol.phase_times()
+    sol.enable_timing(False)
     c1 = sol.counts()
+    w1 = sol.work_counts()
     elapsed = t1 - t0
     # particle-steps: every step advances the lists it starts from; with the
     # forced split the population is constant after the first step
     n1 = c1["n_low_last"] + c1["n_high_last"]
     psteps = args.steps * (n0 + n1) / 2.0
     value = psteps / elapsed
+
+    # the same replays without timing nodes (reported beside value)
+    _log("untimed", args.steps)
+    u0 = time.perf_counter()
+    sol.step_graph(args.steps)
+    sol.synchronize()
+    untimed_ms = (time.perf_counter() - u0) / args.steps * 1e3
 
     # per-step latency (host wall clock, one synchronised step each)
     _log("latency", args.latency_steps)
@@ -132,41 +166,42 @@ def main() -> int:
         lat.append((time.perf_counter() - a) * 1e3)
     lat = np.array(lat)
 
-    # per-phase device time with hipEvents on the solver's stream
-    _log("phase timing")
-    sol.enable_timing(True)
-    sol.step(20)
-    ph = sol.phase_times()
-    sol.enable_timing(False)
-    cnt = sol.counts()
-    work = sol.work_counts()
-    cand = (work["cand_ll"] + work["cand_lh"] + work["cand_hh"] +
-            work["cand_hl"])
-    flops_dens = W_CAND * cand + W_DENS * work["dens_pairs"]
-    flops_force = (W_CAND * cand + W_FORCE * work["force_pairs"] +
-                   W_WALL * work["wall_cand"])
-    dens_tf = flops_dens / (ph[1] * 1e-3) / 1e12
-    force_tf = flops_force / (ph[2] * 1e-3) / 1e12
-    n_now = cnt["n_low"] + cnt["n_high"]
-    cells = cnt["cells_low"] + cnt["cells_high"]
+    # algorithmic work per launch: mean of the counts before and after
+    work = {k: (w0[k] + w1[k]) / 2.0 for k in w0}
+    fl = {
+        "density_low": W_CAND * (work["cand_ll"] + work["cand_lh"]) +
+                       W_DENS * work["dens_pairs_l"],
+        "density_high": W_CAND * (work["cand_hh"] + work["cand_hl"]) +
+                        W_DENS * work["dens_pairs_h"],
+        "force_low": W_CAND * (work["cand_ll"] + work["cand_lh"]) +
+                     W_FORCE * work["force_pairs_l"] + W_WALL * work["wall_l"],
+        "force_high": W_CAND * (work["cand_hh"] + work["cand_hl"]) +
+                      W_FORCE * work["force_pairs_h"] + W_WALL * work["wall_h"],
+    }
+    tflops = {k: fl[k] / (ph[k] * 1e-3) / 1e12 for k in fl if ph[k] > 0}
+    dom = max(fl, key=lambda k: ph[k])
+    kernel_name = {"density_low": "k_density<false>",
+                   "density_high": "k_density<true>",
+                   "force_low": "k_force<false>",
+                   "force_high": "k_force<true>"}[dom]
+    n_now = c1["n_low"] + c1["n_high"]
+    cells = c1["cells_low"] + c1["cells_high"]
     passes = max(1, math.ceil(math.ceil(math.log2(max(cells, 2))) / 8))
     gb = grid_bytes(n_now, cells, passes)
-    grid_gbs = gb / (ph[0] * 1e-3) / 1e9
-    if ph[2] >= ph[1]:
-        dom = ("force", force_tf, flops_force, ph[2])
-    else:
-        dom = ("density", dens_tf, flops_dens, ph[1])
+    grid_gbs = gb / (ph["grid"] * 1e-3) / 1e9
+    traffic = _traffic(kernel_name)
     roofline = {
-        "kernel": f"k_{dom[0]}",
+        "kernel": kernel_name,
         "bound": "valu",
-        "achieved": round(dom[1], 3),
+        "achieved": round(tflops[dom], 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": round(dom[1] / FP32_PEAK_TFLOPS, 4),
-        "frac_nonpacked_peak": round(dom[1] / FP32_NONPACKED_TFLOPS, 4),
-        "traffic": None,
-        "algorithmic_flops_per_launch": int(dom[2]),
-        "avg_launch_ms": round(float(dom[3]), 4),
+        "frac": round(tflops[dom] / FP32_PEAK_TFLOPS, 4),
+        "traffic": traffic["bytes"] if traffic else None,
+        "frac_nonpacked_peak": round(tflops[dom] / FP32_NONPACKED_TFLOPS, 4),
+        "algorithmic_flops_per_launch": int(fl[dom]),
+        "avg_launch_ms": round(ph[dom], 4),
+        "traffic_source": traffic["source"] if traffic else None,
     }
     out = {
         "metric": METRIC,
@@ -186,16 +221,13 @@ def main() -> int:
             "n_low": c1["n_low_last"],
             "n_high": c1["n_high_last"],
             "dt": "CFL 0.4 (FP64 max-reduction on the GPU)",
-            "step": "hipGraph-captured, device-resident counts",
+            "step": "hipGraph-captured pairs of steps, device-resident counts",
         },
+        "untimed_graph_ms_per_step": round(untimed_ms, 4),
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
                             "p99": round(float(np.percentile(lat, 99)), 4)},
-        "phase_ms": {"grid": round(float(ph[0]), 4),
-                     "density": round(float(ph[1]), 4),
-                     "force": round(float(ph[2]), 4),
-                     "adapt": round(float(ph[3]), 4),
-                     "reduce": round(float(ph[4]), 4)},
-        "density_tflops": round(dens_tf, 3),
+        "phase_ms": {k: round(v, 4) for k, v in ph.items()},
+        "kernel_tflops": {k: round(v, 3) for k, v in tflops.items()},
         "roofline": roofline,
         "grid_roofline": {
             "kernel": "grid build (hash + radix sort + ranges + reorder)",
@@ -207,7 +239,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": int(gb),
             "radix_passes": passes,
         },
-        "work": work,
+        "work": {k: int(v) for k, v in work.items()},
     }
     if args.copy_probe_mb > 0:
         import ctypes

@@ -188,18 +188,24 @@ int tss_conservation(tss_solver* s, double out[5], double err_bound[5]);
  * current lists; does not advance. counts/sums hold 5*n entries. */
 int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums,
                   size_t n);
-/* Per-phase device time of the last tss_step call with timing enabled (ms):
- * [0] grid build (hash+sort+ranges+reorder) [1] density [2] force
- * [3] adapt+compaction [4] reductions. */
+/* Device time per phase (ms), averaged over every step taken (eager or
+ * graph-replayed) since timing was switched on, from hipEvents recorded on
+ * the solver's stream around each phase without a host wait per step:
+ * [0] grid build (hash, radix sort, cell ranges, reorder) [1] density, low
+ * targets [2] density, high targets [3] force + integration, low targets
+ * [4] force + integration, high targets [5] FP64 reductions [6] adaptivity
+ * and list compaction [7] finalize (counts, dt). tss_enable_timing resets
+ * the averages. No counterpart in the reference (it times nothing). */
+#define TSS_PHASE_COUNT 8
 int tss_enable_timing(tss_solver* s, int on);
-int tss_phase_times(tss_solver* s, float out_ms[5]);
+int tss_phase_times(tss_solver* s, float out_ms[TSS_PHASE_COUNT]);
 /* Exact work of the density and force kernels on the current state (grid
- * rebuilt from the current lists): [0] low-target same-level candidates,
- * [1] low-target cross-level candidates, [2] high-target same-level
- * candidates, [3] high-target cross-level candidates, [4] density pairs,
- * [5] force pairs, [6] wall candidates, [7] targets. Multiplied by fixed
- * per-item flop weights these give the kernels' algorithmic flops. */
-#define TSS_WORK_COUNT 8
+ * rebuilt from the current lists), per target level: candidates [0] low
+ * same-level [1] low cross-level [2] high same-level [3] high cross-level;
+ * density pairs [4] low [5] high; force pairs [6] low [7] high; wall
+ * candidates [8] low [9] high; targets [10] low [11] high. Multiplied by
+ * fixed per-item flop weights these give each kernel's algorithmic flops. */
+#define TSS_WORK_COUNT 12
 int tss_work_counts(tss_solver* s, unsigned long long out[TSS_WORK_COUNT]);
 /* Replaces WCSPHSolver::~WCSPHSolver (R/WCSPHSolver.cu:2522-2535). */
 void tss_destroy(tss_solver* s);

@@ -155,9 +155,21 @@ struct StepShape {
     int list_parity;          // which list buffer holds the current lists
 };
 
+// Timing points of one step (events recorded before / after each phase):
+// 0 start | grid build | 1 | density low | 2 | density high | 3 | force low
+// | 4 | force high | 5 | reductions | 6 | adaptivity + compaction | 7 |
+// finalize | 8
+constexpr int kTimingPoints = 9;
+// Work counters of k_work: candidates (low same, low cross, high same, high
+// cross), density pairs (low, high), force pairs (low, high), wall
+// candidates (low, high), targets (low, high)
+constexpr int kWorkCounts = 12;
+
 // Kernel launchers (tss_kernels.hip). All enqueue on `stream`.
+// timing_events: hipEvent_t[kTimingPoints] or nullptr; event_flags is passed
+// to hipEventRecordWithFlags (hipEventRecordExternal under graph capture).
 void launch_step(const DevState& d, const StepShape& sh, void* stream,
-                 void* timing_events /* hipEvent_t[6] or nullptr */);
+                 void* timing_events, unsigned int event_flags);
 void launch_neighbors_n(const DevState& d, const StepShape& sh, int scene,
                         int level, int n, uint32_t* counts, uint32_t* sums,
                         void* stream);

@@ -82,6 +82,8 @@ float sqrt_threshold(float R, bool inclusive) {
 }
 
 constexpr double kPi = 3.14159265358979323846;  // R/WCSPHSolver.cu:19
+static_assert(TSS_PHASE_COUNT == tss::kTimingPoints - 1, "phase count");
+static_assert(TSS_WORK_COUNT == tss::kWorkCounts, "work counter count");
 
 // Bulk value (in units of rho0) of the high-level density estimator: the
 // blended kernel (W_h + W_H)/2 summed only over r <= H (R:402-474), i.e.
@@ -193,9 +195,6 @@ struct tss_batch {
     std::vector<SceneHost> scenes;
     RangeAlloc a_low, a_bnd, a_cl, a_ch;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    bool timing = false;
-    hipEvent_t ev[6] = {};
-    float phase_ms[5] = {0, 0, 0, 0, 0};
     int dens_block = 256, force_block = 256;
     long long cap_l = 0, cap_b = 0, ccl = 0, cch = 0;
     int max_scenes = 0;
@@ -205,10 +204,13 @@ struct tss_batch {
 
     ~tss_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (auto& g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+        invalidate_graphs();
+        for (auto& row : tev)
+            for (auto& e : row)
+                if (e) (void)hipEventDestroy(e);
+        for (auto& row : placeholder)
+            for (auto& e : row)
+                if (e) (void)hipEventDestroy(e);
         for (void* p : owned) (void)hipFree(p);
         if (nb_counts) (void)hipFree(nb_counts);
         if (nb_sums) (void)hipFree(nb_sums);
@@ -246,7 +248,6 @@ struct tss_batch {
         cch = ds.cells_high;
         max_scenes = ds.max_scenes;
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (auto& e : ev) HIP_CHECK(hipEventCreate(&e));
         const long long CL = cap_l, U = 5 * CL;
         d.CL = (int)CL;
         d.NB = (int)cap_b;
@@ -386,6 +387,11 @@ struct tss_batch {
                 (void)hipGraphExecDestroy(g);
                 g = nullptr;
             }
+        }
+        for (auto& t : tgraph) {
+            if (t.exec) (void)hipGraphExecDestroy(t.exec);
+            if (t.g) (void)hipGraphDestroy(t.g);
+            t = TimedGraph{};
         }
     }
 
@@ -631,48 +637,148 @@ struct tss_batch {
         update_shape();
     }
 
-    // With timing on, every step records hipEvents around its phases on the
-    // batch's stream and waits for them; phase times accumulate until the
-    // next tss_enable_timing call.
+    // ---- timing ------------------------------------------------------------
+    // With timing on, every step records hipEvents at tss::kTimingPoints
+    // points on the batch's stream into one slot of a ring of kTimingRing
+    // step slots. Nothing waits per step: a slot's times are harvested (its
+    // last event waited for) before the slot is reused, and all pending
+    // slots when the phase times are read.
+    static constexpr int kTimingRing = 64;
+    static constexpr int kPhases = tss::kTimingPoints - 1;
+    bool timing = false;
+    bool tev_made = false;
+    hipEvent_t tev[kTimingRing][tss::kTimingPoints] = {};
+    bool slot_busy[kTimingRing] = {};
+    int slot_next = 0;
+    double phase_sum[kPhases] = {};
+    long long phase_n = 0;
+
+    void harvest(int slot) {
+        if (!slot_busy[slot]) return;
+        slot_busy[slot] = false;
+        HIP_CHECK(hipEventSynchronize(tev[slot][tss::kTimingPoints - 1]));
+        for (int p = 0; p < kPhases; ++p) {
+            float t = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&t, tev[slot][p], tev[slot][p + 1]));
+            phase_sum[p] += t;
+        }
+        ++phase_n;
+    }
+    void harvest_all() {
+        for (int k = 0; k < kTimingRing; ++k) harvest((slot_next + k) % kTimingRing);
+    }
+    int acquire_slot() {
+        const int s = slot_next;
+        slot_next = (slot_next + 1) % kTimingRing;
+        harvest(s);
+        slot_busy[s] = true;
+        return s;
+    }
+    void set_timing(bool on) {
+        harvest_all();
+        if (on && !tev_made) {
+            for (auto& row : tev)
+                for (auto& e : row) HIP_CHECK(hipEventCreate(&e));
+            tev_made = true;
+        }
+        timing = on;
+        for (double& x : phase_sum) x = 0.0;
+        phase_n = 0;
+    }
+    void phase_times(float* out) {
+        harvest_all();
+        for (int p = 0; p < kPhases; ++p)
+            out[p] = phase_n ? (float)(phase_sum[p] / (double)phase_n) : 0.0f;
+    }
+
     void step_eager(int n) {
         for (int i = 0; i < n; ++i) {
-            tss::launch_step(d, sh, stream, timing ? (void*)ev : nullptr);
+            void* ev = timing ? (void*)tev[acquire_slot()] : nullptr;
+            tss::launch_step(d, sh, stream, ev, 0u);
             sh.list_parity ^= 1;
-            if (timing) {
-                HIP_CHECK(hipEventSynchronize(ev[5]));
-                for (int p = 0; p < 5; ++p) {
-                    float t;
-                    HIP_CHECK(hipEventElapsedTime(&t, ev[p], ev[p + 1]));
-                    phase_sum[p] += t;
-                }
-                ++phase_n;
-            }
         }
         HIP_CHECK(hipGetLastError());
     }
 
-    double phase_sum[5] = {0, 0, 0, 0, 0};
-    long long phase_n = 0;
-
     // Two consecutive steps captured once per starting list parity and
     // replayed: no host synchronisation, counts stay on the device.
+    void capture_pair(hipGraph_t* g, hipEvent_t* ev0, hipEvent_t* ev1) {
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        tss::StepShape s2 = sh;
+        const unsigned int fl = ev0 ? hipEventRecordExternal : 0u;
+        tss::launch_step(d, s2, stream, ev0, fl);
+        s2.list_parity ^= 1;
+        tss::launch_step(d, s2, stream, ev1, fl);
+        HIP_CHECK(hipStreamEndCapture(stream, g));
+    }
+
+    // The timed variant holds event-record nodes at every timing point of
+    // both steps; before each replay they are pointed at the events of two
+    // fresh ring slots (hipGraphExecEventRecordNodeSetEvent), so kernel times
+    // are taken inside the replayed graphs themselves.
+    struct TimedGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipGraphNode_t node[2][tss::kTimingPoints] = {};
+    };
+    TimedGraph tgraph[2];
+    hipEvent_t placeholder[2][tss::kTimingPoints] = {};
+
+    void build_timed_graph(int par) {
+        TimedGraph& t = tgraph[par];
+        if (!placeholder[0][0])
+            for (auto& row : placeholder)
+                for (auto& e : row) HIP_CHECK(hipEventCreate(&e));
+        capture_pair(&t.g, placeholder[0], placeholder[1]);
+        size_t n = 0;
+        HIP_CHECK(hipGraphGetNodes(t.g, nullptr, &n));
+        std::vector<hipGraphNode_t> nodes(n);
+        HIP_CHECK(hipGraphGetNodes(t.g, nodes.data(), &n));
+        int found = 0;
+        for (hipGraphNode_t nd : nodes) {
+            hipGraphNodeType ty;
+            HIP_CHECK(hipGraphNodeGetType(nd, &ty));
+            if (ty != hipGraphNodeTypeEventRecord) continue;
+            hipEvent_t e = nullptr;
+            HIP_CHECK(hipGraphEventRecordNodeGetEvent(nd, &e));
+            for (int q = 0; q < 2; ++q)
+                for (int k = 0; k < tss::kTimingPoints; ++k)
+                    if (placeholder[q][k] == e && !t.node[q][k]) {
+                        t.node[q][k] = nd;
+                        ++found;
+                    }
+        }
+        require(found == 2 * tss::kTimingPoints,
+                "graph capture did not keep the timing event nodes");
+        HIP_CHECK(hipGraphInstantiate(&t.exec, t.g, nullptr, nullptr, 0));
+    }
+
     void step_graph(int n) {
         const int pairs = n / 2;
         if (pairs > 0) {
             const int par = sh.list_parity;
-            if (!graph[par]) {
-                hipGraph_t g;
-                HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-                tss::StepShape s2 = sh;
-                tss::launch_step(d, s2, stream, nullptr);
-                s2.list_parity ^= 1;
-                tss::launch_step(d, s2, stream, nullptr);
-                HIP_CHECK(hipStreamEndCapture(stream, &g));
-                hipError_t e = hipGraphInstantiate(&graph[par], g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                HIP_CHECK(e);
+            if (timing) {
+                if (!tgraph[par].exec) build_timed_graph(par);
+                TimedGraph& t = tgraph[par];
+                for (int i = 0; i < pairs; ++i) {
+                    for (int q = 0; q < 2; ++q) {
+                        const int slot = acquire_slot();
+                        for (int k = 0; k < tss::kTimingPoints; ++k)
+                            HIP_CHECK(hipGraphExecEventRecordNodeSetEvent(t.exec, t.node[q][k],
+                                                                          tev[slot][k]));
+                    }
+                    HIP_CHECK(hipGraphLaunch(t.exec, stream));
+                }
+            } else {
+                if (!graph[par]) {
+                    hipGraph_t g;
+                    capture_pair(&g, nullptr, nullptr);
+                    hipError_t e = hipGraphInstantiate(&graph[par], g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    HIP_CHECK(e);
+                }
+                for (int i = 0; i < pairs; ++i) HIP_CHECK(hipGraphLaunch(graph[par], stream));
             }
-            for (int i = 0; i < pairs; ++i) HIP_CHECK(hipGraphLaunch(graph[par], stream));
         }
         if (n & 1) step_eager(1);
         HIP_CHECK(hipGetLastError());
@@ -943,9 +1049,9 @@ struct tss_batch {
         for (auto& x : scenes) alive += x.alive;
         require(alive == 1 && scenes[sid].alive, "work counts need a batch of one scene");
         unsigned long long* dev = nullptr;
-        HIP_CHECK(hipMalloc(&dev, 8 * sizeof(unsigned long long)));
+        HIP_CHECK(hipMalloc(&dev, tss::kWorkCounts * sizeof(unsigned long long)));
         tss::launch_work_counts(d, sh, dev, stream);
-        hipError_t e = hipMemcpyAsync(out, dev, 8 * sizeof(unsigned long long),
+        hipError_t e = hipMemcpyAsync(out, dev, tss::kWorkCounts * sizeof(unsigned long long),
                                       hipMemcpyDeviceToHost, stream);
         hipError_t e2 = hipStreamSynchronize(stream);
         (void)hipFree(dev);
@@ -1162,17 +1268,14 @@ int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums, si
 int tss_enable_timing(tss_solver* s, int on) {
     return guarded([&] {
         require(s, "null argument");
-        s->b.timing = on != 0;
-        for (double& x : s->b.phase_sum) x = 0.0;
-        s->b.phase_n = 0;
+        s->b.set_timing(on != 0);
     });
 }
 
-int tss_phase_times(tss_solver* s, float out[5]) {
+int tss_phase_times(tss_solver* s, float out[TSS_PHASE_COUNT]) {
     return guarded([&] {
         require(s && out, "null argument");
-        for (int i = 0; i < 5; ++i)
-            out[i] = s->b.phase_n ? (float)(s->b.phase_sum[i] / (double)s->b.phase_n) : 0.0f;
+        s->b.phase_times(out);
     });
 }
 

@@ -1046,15 +1046,16 @@ __global__ void k_neighbors(DevState d, const uint32_t* __restrict__ vals,
 __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
     const int nl = d.counts->n_l;
     const int n = nl + d.counts->n_h;
-    unsigned long long w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long w[kWorkCounts] = {};
     GRID_STRIDE(k, n) {
         const int s = d.sscene[k];
         const SceneDev& sc = d.scenes[s];
         const float2 xi = d.spos[k];
         const uint8_t si = d.sstate[k];
         const bool high = k >= nl;
+        const int lv = high ? 1 : 0;
         const bool i_prom = !(si & 1u);
-        w[7] += 1;
+        w[10 + lv] += 1;
         // same level (rows skipped exactly as the neighbour kernels do)
         {
             Box b = scan_box(sc, high, xi, high ? sc.H : sc.h);
@@ -1068,8 +1069,8 @@ __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
                     const uint8_t sj = d.sstate[j];
                     w[high ? 2 : 0] += 1;
                     bool lam = high ? !(i_prom && (sj & 1u)) : true;
-                    if (lam && r2 <= (high ? sc.t_le_H : sc.t_le_h)) w[4] += 1;
-                    if (lam && r2 != 0.0f && r2 <= (high ? sc.t_lt_H : sc.t_lt_h)) w[5] += 1;
+                    if (lam && r2 <= (high ? sc.t_le_H : sc.t_le_h)) w[4 + lv] += 1;
+                    if (lam && r2 != 0.0f && r2 <= (high ? sc.t_lt_H : sc.t_lt_h)) w[6 + lv] += 1;
                 }
             }
         }
@@ -1086,9 +1087,9 @@ __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
                     const uint8_t sj = d.sstate[j];
                     w[high ? 3 : 1] += 1;
                     bool lam = high ? !((si & 1u) && (sj & 1u)) : !(sj & 1u);
-                    if (lam && r2 <= (high ? sc.t_le_H : sc.t_le_h)) w[4] += 1;
+                    if (lam && r2 <= (high ? sc.t_le_H : sc.t_le_h)) w[4 + lv] += 1;
                     if (lam && r2 != 0.0f && r2 <= sc.t_lt_h && (!high || sj != kDeleted))
-                        w[5] += 1;
+                        w[6 + lv] += 1;
                 }
             }
         }
@@ -1098,11 +1099,11 @@ __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
             for (int cy = b.y0; cy <= b.y1; ++cy) {
                 int st, en;
                 row_run(d.bstart, d.bend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1, st, en);
-                if (en > st) w[6] += (unsigned long long)(en - st);
+                if (en > st) w[8 + lv] += (unsigned long long)(en - st);
             }
         }
     }
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < kWorkCounts; ++q) {
         unsigned long long v = w[q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
         if ((threadIdx.x & 63) == 0 && v) atomicAdd(&out[q], v);
@@ -1214,23 +1215,30 @@ void launch_grid_only(const DevState& d, const StepShape& sh, void* stream) {
 }
 
 void launch_step(const DevState& d, const StepShape& sh, void* stream,
-                 void* timing_events) {
+                 void* timing_events, unsigned int event_flags) {
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t* ev = (hipEvent_t*)timing_events;
+    // event points: see kTimingPoints (tss_device.h)
+    auto mark = [&](int k) {
+        if (ev) (void)hipEventRecordWithFlags(ev[k], st, event_flags);
+    };
     const int par = sh.list_parity;
-    if (ev) (void)hipEventRecord(ev[0], st);
+    mark(0);
     const int cur = enqueue_grid(d, sh, st);
     const uint32_t* vals = d.vals[cur];
-    if (ev) (void)hipEventRecord(ev[1], st);
+    mark(1);
     const int g_low = blocks_for(d.CL, kPairBlock, 1 << 30);
     const int g_high = blocks_for(4LL * d.CL, kPairBlock, 1 << 30);
     k_density<false><<<g_low, kPairBlock, 0, st>>>(d, vals);
+    mark(2);
     k_density<true><<<g_high, kPairBlock, 0, st>>>(d, vals);
-    if (ev) (void)hipEventRecord(ev[2], st);
+    mark(3);
     k_force<false><<<g_low, kPairBlock, 0, st>>>(d, vals);
+    mark(4);
     k_force<true><<<g_high, kPairBlock, 0, st>>>(d, vals);
-    if (ev) (void)hipEventRecord(ev[3], st);
+    mark(5);
     k_reduce<<<sh.red_slots, 256, 0, st>>>(d, vals);
+    mark(6);
     k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
     k_groups<<<blocks_for(d.CL, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
     k_cap<<<sh.S, 256, 0, st>>>(d);
@@ -1241,9 +1249,9 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     k_compact_scatter<<<ntiles, kScanThreads, 0, st>>>(
         d, sh, vals, d.list_g[par], d.list_l[par ^ 1], d.list_h[par ^ 1],
         d.list_g[par ^ 1]);
-    if (ev) (void)hipEventRecord(ev[4], st);
+    mark(7);
     k_finalize<<<sh.S, 256, 0, st>>>(d);
-    if (ev) (void)hipEventRecord(ev[5], st);
+    mark(8);
 }
 
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
@@ -1257,7 +1265,7 @@ void launch_work_counts(const DevState& d, const StepShape& sh,
                         unsigned long long* out, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     enqueue_grid(d, sh, st);
-    (void)hipMemsetAsync(out, 0, 8 * sizeof(unsigned long long), st);
+    (void)hipMemsetAsync(out, 0, kWorkCounts * sizeof(unsigned long long), st);
     k_work<<<sh.grid_elem, 256, 0, st>>>(d, out);
 }
 

@@ -19,6 +19,15 @@ _DTYPES = {
 }
 
 
+# tss_phase_times order (include/tss.h)
+PHASES = ["grid", "density_low", "density_high", "force_low", "force_high",
+          "reduce", "adapt", "finalize"]
+# tss_work_counts order
+WORK = ["cand_ll", "cand_lh", "cand_hh", "cand_hl", "dens_pairs_l",
+        "dens_pairs_h", "force_pairs_l", "force_pairs_h", "wall_l", "wall_h",
+        "targets_l", "targets_h"]
+
+
 def _field_shape(name: str, cnt: dict) -> tuple[np.dtype, tuple]:
     """dtype and shape of a field for the current counts."""
     if name in ("DT",):
@@ -147,19 +156,20 @@ class Solver:
         N.check(N.lib().tss_enable_timing(self._h, 1 if on else 0),
                 "tss_enable_timing")
 
-    def phase_times(self) -> np.ndarray:
-        out = np.zeros(5, np.float32)
+    def phase_times(self) -> dict:
+        """Mean device ms per phase over the steps since enable_timing."""
+        out = np.zeros(len(PHASES), np.float32)
         N.check(N.lib().tss_phase_times(self._h, N.ptr(out)),
                 "tss_phase_times")
-        return out
+        return {k: float(v) for k, v in zip(PHASES, out)}
 
     def work_counts(self) -> dict:
-        out = np.zeros(8, np.uint64)
+        """Exact candidate / pair / wall / target counts of the density and
+        force kernels on the current state, per target level."""
+        out = np.zeros(len(WORK), np.uint64)
         N.check(N.lib().tss_work_counts(self._h, N.ptr(out)),
                 "tss_work_counts")
-        keys = ["cand_ll", "cand_lh", "cand_hh", "cand_hl", "dens_pairs",
-                "force_pairs", "wall_cand", "targets"]
-        return {k: int(v) for k, v in zip(keys, out)}
+        return {k: int(v) for k, v in zip(WORK, out)}
 
     def snapshot(self) -> dict:
         """Every by-id field, the lists and dt (synchronising)."""
