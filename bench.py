@@ -96,6 +96,86 @@ def cpu_baseline(snap: dict, sc, steps: int, threads: int) -> dict:
             "seconds": round(dt, 3)}
 
 
+def run_batched(args, rule: str) -> dict:
+    """Batched scenes: LogU[2k,64k] two-scale dam breaks of random aspect and
+    viscosity advanced together (one launch per phase, graph-replayed steps)
+    against the same solver run scene-at-a-time."""
+    from twoscalesplit_amd import Batch, Solver, scenes
+    scs = scenes.c4_batched(n_scenes=args.batch_scenes,
+                            steps=args.batch_steps, rule=rule)
+    cap = sum(s.cap_low for s in scs)
+    nb = sum(len(s.bnd_xy) for s in scs)
+    cl = sum(s.cells[0] for s in scs)
+    ch = sum(s.cells[1] for s in scs)
+    _log("batched:", len(scs), "scenes,", cap, "low particles")
+    b = Batch(len(scs), cap, nb, cl, ch)
+    ids = [b.add_scene(s.cfg, s.low_xy, s.cap_low, s.bnd_xy) for s in scs]
+    b.synchronize()
+    # the autotuner's trials are the run's first steps
+    t0 = time.perf_counter()
+    tune = b.autotune(steps_per_trial=1, rounds=2)
+    b.synchronize()
+    t1 = time.perf_counter()
+    rest = args.batch_steps - tune["steps_taken"]
+    b.step_graph(rest)
+    b.synchronize()
+    t2 = time.perf_counter()
+    cnt = [b.counts(i) for i in ids]
+    ps_total = sum(c["particle_steps"] for c in cnt)
+    # post-tuning particle-steps: total minus the tuning steps' share
+    lat = []
+    for _ in range(args.latency_steps):
+        a = time.perf_counter()
+        b.step(1)
+        b.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    cnt2 = [b.counts(i) for i in ids]
+    ps_lat = sum(c["particle_steps"] for c in cnt2) - ps_total
+    lat = np.array(lat)
+    n_low = sum(c["n_low"] for c in cnt)
+    n_high = sum(c["n_high"] for c in cnt)
+    b.close()
+
+    # scene-at-a-time: the same single-scene solver (graph-replayed steps),
+    # one scene after the other, same step count
+    _log("scene-at-a-time baseline")
+    ps_one, t_one = 0, 0.0
+    for s in scs:
+        sol = Solver(s.cfg, s.low_xy, s.cap_low, s.bnd_xy)
+        sol.synchronize()
+        a = time.perf_counter()
+        sol.step_graph(args.batch_steps)
+        sol.synchronize()
+        t_one += time.perf_counter() - a
+        ps_one += sol.counts()["particle_steps"]
+        sol.close()
+    value = ps_total / (t2 - t0)
+    base = ps_one / t_one
+    return {
+        "workload": f"{len(scs)} two-scale dam breaks, low counts LogU[2k,64k], "
+                    f"aspect U(0.25,2), alpha U(0.02,0.1), high cap 4x low, "
+                    f"{args.batch_steps} steps each, "
+                    + ("surface-band injection/removal" if rule == "band" else
+                       "reference split rule at mid-block"),
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "ms_per_step": round((t2 - t0) / args.batch_steps * 1e3, 4),
+        "post_tuning_ms_per_step": round((t2 - t1) / max(rest, 1) * 1e3, 4),
+        "autotune_steps": tune["steps_taken"],
+        "tuning": tune["tuning"],
+        "n_low": n_low,
+        "n_high_at_end": n_high,
+        "particle_steps": ps_total,
+        "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                            "p99": round(float(np.percentile(lat, 99)), 4)},
+        "latency_particle_steps_per_step": round(ps_lat / max(len(lat), 1)),
+        "scene_at_a_time": {"value": round(base, 1),
+                            "seconds": round(t_one, 3),
+                            "particle_steps": ps_one},
+        "speedup_vs_scene_at_a_time": round(value / base, 3),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +186,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--natural", action="store_true",
                     help="surface-band C3 instead of the forced all-split")
+    ap.add_argument("--batch-scenes", type=int, default=256,
+                    help="scenes of the batched dam-break mix (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=16,
+                    help="requests of the served run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=16.0)
     ap.add_argument("--copy-probe-mb", type=int, default=512,
                     help="plain-copy bandwidth probe size (0: skip); also "
                          "the reference dispatch for HBM counter calibration")
@@ -143,10 +229,8 @@ def main() -> int:
     c1 = sol.counts()
     w1 = sol.work_counts()
     elapsed = t1 - t0
-    # particle-steps: every step advances the lists it starts from; with the
-    # forced split the population is constant after the first step
-    n1 = c1["n_low_last"] + c1["n_high_last"]
-    psteps = args.steps * (n0 + n1) / 2.0
+    # particle-steps advanced in the timed region (device-side counter)
+    psteps = c1["particle_steps"] - c0["particle_steps"]
     value = psteps / elapsed
 
     # the same replays without timing nodes (reported beside value)
@@ -248,14 +332,23 @@ def main() -> int:
         _native.check(_native.lib().tss_bandwidth_probe(
             args.copy_probe_mb << 20, 3, ctypes.byref(g)), "bandwidth probe")
         out["copy_probe_gbs"] = round(g.value, 1)
-    if not args.no_cpu_baseline:
+    snap = None if args.no_cpu_baseline else sol.snapshot()
+    sol.close()
+    if args.batch_scenes > 0:
+        out["batched"] = run_batched(args, "band")
+        out["batched_reference_rule"] = run_batched(args, "reference")
+    if args.serve_requests > 0:
+        from twoscalesplit_amd import serve
+        _log("served:", args.serve_requests, "requests at",
+             args.serve_rate, "/s")
+        out["served"] = serve.run_rate(scenes.SEED, args.serve_rate,
+                                       args.serve_requests, 8_000_000)
+    if snap is not None:
         threads = min(16, os.cpu_count() or 1)
         _log("cpu baseline", args.cpu_steps, "steps,", threads, "threads")
-        snap = sol.snapshot()
         out["cpu_baseline"] = cpu_baseline(snap, sc, args.cpu_steps, threads)
     else:
         out["cpu_baseline"] = None
-    sol.close()
     print(json.dumps(out, default=float), flush=True)
     return 0
 

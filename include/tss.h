@@ -134,6 +134,8 @@ typedef struct tss_counts_t {
     int step;                             /* completed steps                 */
     int cap_low, n_bnd;
     int cells_low, cells_high;
+    long long particle_steps;             /* low + high particles advanced,
+                                             summed over completed steps    */
 } tss_counts_t;
 
 /* ---- configuration ---------------------------------------------------- */
@@ -207,6 +209,27 @@ int tss_phase_times(tss_solver* s, float out_ms[TSS_PHASE_COUNT]);
  * fixed per-item flop weights these give each kernel's algorithmic flops. */
 #define TSS_WORK_COUNT 12
 int tss_work_counts(tss_solver* s, unsigned long long out[TSS_WORK_COUNT]);
+/* Launch shapes of the four neighbour kernels, index order: [0] density low
+ * targets, [1] density high, [2] force low, [3] force high. block: 128, 256
+ * or 512 threads; depth: LDS buffer depth, 8, 16 or 32 pairs per lane;
+ * xcd_order: 1 gives each of the 8 XCDs one contiguous range of blocks;
+ * grid: 0 = one thread per target of capacity, else a persistent grid of at
+ * most this many blocks. Results never depend on the shapes (every
+ * neighbour sum is lane-local). Default: 256, 16, 0, 0. No counterpart in
+ * the reference (fixed 256-thread blocks, R/WCSPHSolver.cu:2421). */
+typedef struct tss_tuning {
+    int block[4];
+    int depth[4];
+    int xcd_order[4];
+    int grid[4];
+} tss_tuning;
+/* Launch shapes and autotuner of a single-scene solver (as for batches,
+ * tss_batch_autotune below). */
+int tss_get_tuning(tss_solver* s, tss_tuning* out);
+int tss_set_tuning(tss_solver* s, const tss_tuning* t);
+int tss_autotune(tss_solver* s, int steps_per_trial, int rounds,
+                 tss_tuning* out, int* steps_taken, float* trial_log,
+                 int log_rows, int* n_log);
 /* Replaces WCSPHSolver::~WCSPHSolver (R/WCSPHSolver.cu:2522-2535). */
 void tss_destroy(tss_solver* s);
 
@@ -241,9 +264,71 @@ int tss_batch_conservation(tss_batch* b, int scene_id, double out[5],
                            double err_bound[5]);
 int tss_batch_synchronize(tss_batch* b);
 int tss_batch_num_scenes(tss_batch* b, int* n);
-/* Launch-shape tuning: block sizes of the neighbour kernels. */
-int tss_batch_set_tuning(tss_batch* b, int density_block, int force_block);
+int tss_batch_get_tuning(tss_batch* b, tss_tuning* out);
+int tss_batch_set_tuning(tss_batch* b, const tss_tuning* t);
+/* Autotuner: times candidate shapes with hipEvents while advancing the
+ * batch by real steps (they count, the results being shape-independent):
+ * stage 1 tries every block x depth, stage 2 the XCD order and persistent
+ * grid caps (4 and 8 blocks per CU) on each kernel's stage-1 winner; each
+ * candidate runs `rounds` times for steps_per_trial steps, interleaved, and
+ * the fastest total per kernel is kept (and returned in out). steps_taken:
+ * steps advanced. trial_log (optional): log_rows x 8 floats per trial,
+ * (block, depth, xcd_order, grid, ms of kernels 0..3); n_log rows used. */
+int tss_batch_autotune(tss_batch* b, int steps_per_trial, int rounds,
+                       tss_tuning* out, int* steps_taken, float* trial_log,
+                       int log_rows, int* n_log);
 void tss_batch_destroy(tss_batch* b);
+
+/* ---- serving loop ---------------------------------------------------------- */
+/* A scene request: its initial state, the steps it asks for and its arrival
+ * time; optional output buffers receive fields of its final state. No
+ * counterpart in the reference (one scene per process, driven per frame by
+ * R/main.cpp:70). */
+typedef struct tss_request {
+    const tss_config* cfg;
+    const float* low_xy;
+    int n_low, cap_low;
+    const float* bnd_xy;
+    int n_bnd;
+    int steps;                      /* steps to advance (>= 1)            */
+    double arrival_s;               /* seconds after the start of the run */
+    void* out[TSS_FIELD_COUNT];     /* optional: field copies at finish   */
+    size_t out_bytes[TSS_FIELD_COUNT];
+    /* results */
+    int status;                     /* TSS_OK or the error that rejected it */
+    double admit_s, finish_s;       /* seconds after the start of the run */
+    long long particle_steps;
+} tss_request;
+
+typedef struct tss_serve_stats {
+    double wall_s;                  /* start of the run to the last finish */
+    double busy_s;                  /* time with at least one scene resident */
+    long long particle_steps;
+    double particle_steps_per_s;    /* particle_steps / wall_s            */
+    int ticks, graph_ticks;         /* batch advances; of them graph pairs */
+    int max_resident;
+    double step_ms_p50, step_ms_p99;        /* per step, host wall clock  */
+    double request_ms_p50, request_ms_p99;  /* arrival to finish          */
+    double queue_ms_p50, queue_ms_p99;      /* arrival to admission       */
+    long long batch_bytes;          /* device memory of the serving batch */
+    long long budget_bytes;
+    long long peak_admitted_bytes;  /* budget accounting of resident scenes */
+} tss_serve_stats;
+
+/* Continuous batching: runs the requests to completion on one batch of the
+ * given capacities, in real time. Every tick it admits the requests that
+ * have arrived, in arrival order, whenever the batch has room and the sum of
+ * the resident scenes' device bytes stays within budget_bytes (a request
+ * that does not fit waits; later ones may pass it); advances every resident
+ * scene together (one launch per phase; hipGraph pairs of steps while the
+ * resident set is unchanged); and retires each scene the step it completes
+ * its steps, copying the requested fields out first. Results of a scene do
+ * not depend on the others resident with it. Requests that can never fit
+ * (or are invalid) get a nonzero status; the run continues. */
+int tss_serve(const tss_batch_desc* capacity, long long budget_bytes,
+              tss_request* requests, int n_requests, tss_serve_stats* stats);
+/* Device bytes a scene accounts for in admission (its share of a batch). */
+long long tss_scene_bytes(const tss_config* cfg, int cap_low, int n_bnd);
 
 /* ---- scene generator (host) ----------------------------------------------- */
 /* Seeded dam-break scene: fluid block of nx*ny particles on a lattice of

@@ -193,3 +193,27 @@ def test_generator_rejects_bad_arguments():
                                      N.ptr(low), None)
     assert rc != N.TSS_OK
     assert N.lib().tss_canvas_count(-1, 4, 1) == -1
+
+
+def test_header_is_plain_c99():
+    import shutil
+    import subprocess
+    import tempfile
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    with tempfile.TemporaryDirectory() as tmp:
+        src = os.path.join(tmp, "h.c")
+        with open(src, "w") as f:
+            f.write('#include "tss.h"\nint main(void) { return 0; }\n')
+        r = subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-pedantic",
+                            "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                            src], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+def test_tuning_struct_layout():
+    t = N.TssTuning.uniform(128, 8, 1, 1024)
+    assert C.sizeof(t) == 64
+    d = t.as_dict()
+    assert d["force_high"] == {"block": 128, "depth": 8, "xcd_order": 1, "grid": 1024}

@@ -66,7 +66,63 @@ class TssCounts(C.Structure):
         "n_low", "n_high", "n_groups", "n_low_last", "n_high_last", "n_split",
         "n_promote", "n_merge", "n_dissolve", "step", "cap_low", "n_bnd",
         "cells_low",
-        "cells_high")]
+        "cells_high")] + [("particle_steps", C.c_longlong)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class TssTuning(C.Structure):
+    """Launch shapes of the neighbour kernels [density low, density high,
+    force low, force high]."""
+    _fields_ = [("block", C.c_int * 4), ("depth", C.c_int * 4),
+                ("xcd_order", C.c_int * 4), ("grid", C.c_int * 4)]
+
+    KERNELS = ("density_low", "density_high", "force_low", "force_high")
+
+    def as_dict(self) -> dict:
+        return {k: {"block": self.block[i], "depth": self.depth[i],
+                    "xcd_order": self.xcd_order[i], "grid": self.grid[i]}
+                for i, k in enumerate(self.KERNELS)}
+
+    @classmethod
+    def uniform(cls, block=256, depth=16, xcd_order=0, grid=0) -> "TssTuning":
+        t = cls()
+        for i in range(4):
+            t.block[i], t.depth[i] = block, depth
+            t.xcd_order[i], t.grid[i] = xcd_order, grid
+        return t
+
+
+class TssRequest(C.Structure):
+    """One scene request of tss_serve."""
+    _fields_ = [
+        ("cfg", C.POINTER(TssConfig)),
+        ("low_xy", C.c_void_p), ("n_low", C.c_int), ("cap_low", C.c_int),
+        ("bnd_xy", C.c_void_p), ("n_bnd", C.c_int),
+        ("steps", C.c_int),
+        ("arrival_s", C.c_double),
+        ("out", C.c_void_p * len(FIELDS)),
+        ("out_bytes", C.c_size_t * len(FIELDS)),
+        ("status", C.c_int),
+        ("admit_s", C.c_double), ("finish_s", C.c_double),
+        ("particle_steps", C.c_longlong),
+    ]
+
+
+class TssServeStats(C.Structure):
+    _fields_ = [
+        ("wall_s", C.c_double), ("busy_s", C.c_double),
+        ("particle_steps", C.c_longlong),
+        ("particle_steps_per_s", C.c_double),
+        ("ticks", C.c_int), ("graph_ticks", C.c_int),
+        ("max_resident", C.c_int),
+        ("step_ms_p50", C.c_double), ("step_ms_p99", C.c_double),
+        ("request_ms_p50", C.c_double), ("request_ms_p99", C.c_double),
+        ("queue_ms_p50", C.c_double), ("queue_ms_p99", C.c_double),
+        ("batch_bytes", C.c_longlong), ("budget_bytes", C.c_longlong),
+        ("peak_admitted_bytes", C.c_longlong),
+    ]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -102,6 +158,10 @@ PROTOTYPES = {
     "tss_enable_timing": (_I, [_P, _I]),
     "tss_phase_times": (_I, [_P, _P]),
     "tss_work_counts": (_I, [_P, _P]),
+    "tss_get_tuning": (_I, [_P, C.POINTER(TssTuning)]),
+    "tss_set_tuning": (_I, [_P, C.POINTER(TssTuning)]),
+    "tss_autotune": (_I, [_P, _I, _I, C.POINTER(TssTuning), C.POINTER(_I), _P,
+                          _I, C.POINTER(_I)]),
     "tss_destroy": (None, [_P]),
     "tss_batch_memory_bytes": (_I, [C.POINTER(TssBatchDesc),
                                     C.POINTER(C.c_longlong)]),
@@ -116,8 +176,14 @@ PROTOTYPES = {
     "tss_batch_conservation": (_I, [_P, _I, _P, _P]),
     "tss_batch_synchronize": (_I, [_P]),
     "tss_batch_num_scenes": (_I, [_P, C.POINTER(_I)]),
-    "tss_batch_set_tuning": (_I, [_P, _I, _I]),
+    "tss_batch_get_tuning": (_I, [_P, C.POINTER(TssTuning)]),
+    "tss_batch_set_tuning": (_I, [_P, C.POINTER(TssTuning)]),
+    "tss_batch_autotune": (_I, [_P, _I, _I, C.POINTER(TssTuning),
+                                C.POINTER(_I), _P, _I, C.POINTER(_I)]),
     "tss_batch_destroy": (None, [_P]),
+    "tss_serve": (_I, [C.POINTER(TssBatchDesc), C.c_longlong,
+                       C.POINTER(TssRequest), _I, C.POINTER(TssServeStats)]),
+    "tss_scene_bytes": (C.c_longlong, [C.POINTER(TssConfig), _I, _I]),
     "tss_canvas_count": (_I, [_I, _I, _I]),
     "tss_scene_dam_break": (_I, [C.c_uint64, C.c_uint64, _I, _I, _F, _F, _I,
                                  _I, _I, _P, _P]),

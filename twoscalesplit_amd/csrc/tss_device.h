@@ -83,6 +83,8 @@ struct SceneCount {
     // compaction prefixes at the scene's segment ends (kept low, kept high,
     // admitted splits): counts = end - begin, no hot atomics
     int kl0, kl1, kh0, kh1, sp0, sp1;
+    int pad0;
+    unsigned long long psteps;  // particle-steps advanced since admission
 };
 
 struct DevCounts {
@@ -142,6 +144,15 @@ struct DevState {
     uint8_t* cflag;           // [2 (CCL + CCH)] cflag[2c + s] = 1: state s in cell c
 };
 
+// Launch shape of one neighbour kernel: block size (128, 256, 512), LDS
+// buffer depth in pairs per lane (8, 16, 32), XCD-aware block order (0/1),
+// grid cap in blocks (0: capacity-sized grid; else a persistent grid).
+struct PairTune {
+    int block, depth, swz, grid;
+};
+enum { kTuneDensityLow = 0, kTuneDensityHigh, kTuneForceLow, kTuneForceHigh,
+       kPairKernels };
+
 // Host-known launch parameters of one step.
 struct StepShape {
     int passes;               // radix passes
@@ -149,7 +160,7 @@ struct StepShape {
     int grid_elem;            // blocks of element-wise kernels
     int sort_tiles;           // max sort tiles (capacity-based)
     int scan_tiles_l, scan_tiles_h, scan_tiles_g;  // capacity-based tiles
-    int dens_block, force_block;
+    PairTune tune[kPairKernels];
     int S;
     int red_slots;            // chunk slots in use (grid of the reduction)
     int list_parity;          // which list buffer holds the current lists

@@ -7,9 +7,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -17,6 +20,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tss.h"
@@ -188,6 +192,9 @@ T* dalloc(size_t n, std::vector<void*>& owned) {
 // Batch
 // ---------------------------------------------------------------------------
 struct tss_batch {
+    static constexpr long long kElemGridCap = 32768;  // blocks of 256
+    static constexpr long long kPairGridFull = 16384;
+    static constexpr int kPairGridCap = 8192;
     tss::DevState d{};
     tss::StepShape sh{};
     hipStream_t stream = nullptr;
@@ -195,7 +202,8 @@ struct tss_batch {
     std::vector<SceneHost> scenes;
     RangeAlloc a_low, a_bnd, a_cl, a_ch;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int dens_block = 256, force_block = 256;
+    tss::PairTune tune[tss::kPairKernels] = {
+        {256, 16, 0, 0}, {256, 16, 0, 0}, {256, 16, 0, 0}, {256, 16, 0, 0}};
     long long cap_l = 0, cap_b = 0, ccl = 0, cch = 0;
     int max_scenes = 0;
     uint32_t* nb_counts = nullptr;  // neighbour inspection scratch
@@ -333,9 +341,17 @@ struct tss_batch {
         a_cl.reset(ccl);
         a_ch.reset(cch);
         scenes.assign(max_scenes, SceneHost{});
-        sh.grid_elem = (int)std::max<long long>(1, (U + 255) / 256);
-        sh.dens_block = dens_block;
-        sh.force_block = force_block;
+        // element-wise kernels loop over the device-side counts, so their
+        // grid is capped: a batch sized well above its live population
+        // (serving) does not launch idle blocks by the hundred thousand
+        sh.grid_elem = (int)std::min<long long>(kElemGridCap, std::max<long long>(1, (U + 255) / 256));
+        // likewise the neighbour kernels default to a persistent grid when
+        // a capacity-sized grid would exceed kPairGridFull blocks
+        for (int k = 0; k < tss::kPairKernels; ++k) {
+            const long long n = (k == tss::kTuneDensityHigh || k == tss::kTuneForceHigh) ? 4 * CL : CL;
+            if ((n + tune[k].block - 1) / tune[k].block > kPairGridFull) tune[k].grid = kPairGridCap;
+            sh.tune[k] = tune[k];
+        }
         sh.S = max_scenes;
         sh.list_parity = 0;
         update_shape();
@@ -376,8 +392,7 @@ struct tss_batch {
         while ((1LL << bits) < maxkey) ++bits;
         sh.key_bits = bits;
         sh.passes = (bits + 7) / 8;
-        sh.dens_block = dens_block;
-        sh.force_block = force_block;
+        for (int k = 0; k < tss::kPairKernels; ++k) sh.tune[k] = tune[k];
         invalidate_graphs();
     }
 
@@ -784,6 +799,104 @@ struct tss_batch {
         HIP_CHECK(hipGetLastError());
     }
 
+    // ---- launch-shape tuning ------------------------------------------------
+    static void validate_tune(const tss::PairTune& t) {
+        require(t.block == 128 || t.block == 256 || t.block == 512,
+                "tuning: block must be 128, 256 or 512");
+        require(t.depth == 8 || t.depth == 16 || t.depth == 32,
+                "tuning: depth must be 8, 16 or 32");
+        require(t.swz == 0 || t.swz == 1, "tuning: xcd_order must be 0 or 1");
+        require(t.grid >= 0, "tuning: grid must be >= 0");
+    }
+    void set_tuning(const tss::PairTune* t) {
+        for (int k = 0; k < tss::kPairKernels; ++k) validate_tune(t[k]);
+        for (int k = 0; k < tss::kPairKernels; ++k) tune[k] = t[k];
+        update_shape();
+    }
+
+    // Project-written autotuner. Each trial applies one candidate shape to
+    // all four neighbour kernels, advances the batch `steps` real steps
+    // (results do not depend on the shape, so the steps count) and reads
+    // each kernel's mean time from the phase events. Candidates are visited
+    // `rounds` times in interleaved order; the fastest total per kernel wins.
+    // Stage 1: block x depth with the plain grid; stage 2: XCD block order x
+    // persistent grid caps on top of each kernel's stage-1 winner.
+    int autotune(int steps, int rounds, float* trial_log, int log_cap, int* n_log) {
+        require(steps >= 1 && rounds >= 1, "autotune: steps and rounds must be >= 1");
+        int alive = 0;
+        for (auto& x : scenes) alive += x.alive;
+        require(alive > 0, "autotune: no scene");
+        const bool was_timing = timing;
+        int dev = 0, cus = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        int taken = 0, logged = 0;
+        auto run_trials = [&](const std::vector<std::array<tss::PairTune, tss::kPairKernels>>& cand,
+                              std::vector<std::array<double, tss::kPairKernels>>& tot) {
+            tot.assign(cand.size(), {0.0, 0.0, 0.0, 0.0});
+            for (int r = 0; r < rounds; ++r) {
+                for (size_t c = 0; c < cand.size(); ++c) {
+                    set_tuning(cand[c].data());
+                    set_timing(true);
+                    step_eager(steps);
+                    taken += steps;
+                    float ph[kPhases];
+                    phase_times(ph);
+                    for (int k = 0; k < tss::kPairKernels; ++k) tot[c][k] += ph[1 + k];
+                    if (trial_log && logged < log_cap) {
+                        float* row = trial_log + (size_t)logged * 8;
+                        const auto& t = cand[c][0];
+                        row[0] = (float)t.block;
+                        row[1] = (float)t.depth;
+                        row[2] = (float)t.swz;
+                        row[3] = (float)t.grid;
+                        for (int k = 0; k < tss::kPairKernels; ++k) row[4 + k] = ph[1 + k];
+                        ++logged;
+                    }
+                }
+            }
+        };
+        // stage 1
+        std::vector<std::array<tss::PairTune, tss::kPairKernels>> cand;
+        for (int b : {128, 256, 512})
+            for (int dp : {8, 16, 32}) {
+                std::array<tss::PairTune, tss::kPairKernels> a;
+                for (auto& t : a) t = {b, dp, 0, 0};
+                cand.push_back(a);
+            }
+        std::vector<std::array<double, tss::kPairKernels>> tot;
+        run_trials(cand, tot);
+        tss::PairTune best[tss::kPairKernels];
+        double best_t[tss::kPairKernels];
+        for (int k = 0; k < tss::kPairKernels; ++k) {
+            size_t bi = 0;
+            for (size_t c = 1; c < cand.size(); ++c)
+                if (tot[c][k] < tot[bi][k]) bi = c;
+            best[k] = cand[bi][k];
+            best_t[k] = tot[bi][k];
+        }
+        // stage 2 (the plain-grid, plain-order variant is stage 1's winner)
+        cand.clear();
+        for (int swz : {0, 1})
+            for (int gcap : {0, 4 * cus, 8 * cus}) {
+                if (swz == 0 && gcap == 0) continue;
+                std::array<tss::PairTune, tss::kPairKernels> a;
+                for (int k = 0; k < tss::kPairKernels; ++k) a[k] = {best[k].block, best[k].depth, swz, gcap};
+                cand.push_back(a);
+            }
+        run_trials(cand, tot);
+        for (int k = 0; k < tss::kPairKernels; ++k)
+            for (size_t c = 0; c < cand.size(); ++c)
+                if (tot[c][k] < best_t[k]) {
+                    best_t[k] = tot[c][k];
+                    best[k] = cand[c][k];
+                }
+        set_tuning(best);
+        set_timing(was_timing);
+        if (n_log) *n_log = logged;
+        return taken;
+    }
+
     // ---- field I/O -------------------------------------------------------
     struct FieldView {
         void* dev;          // device base of the scene's slice
@@ -1007,6 +1120,7 @@ struct tss_batch {
         o->n_bnd = s.nb;
         o->cells_low = s.ncl;
         o->cells_high = s.nch;
+        o->particle_steps = (long long)sc.psteps;
     }
 
     void conservation(int sid, double* out, double* err) {
@@ -1375,18 +1489,270 @@ int tss_batch_num_scenes(tss_batch* b, int* n) {
     });
 }
 
-int tss_batch_set_tuning(tss_batch* b, int dens_block, int force_block) {
+static void tune_to_abi(const tss::PairTune* t, tss_tuning* o) {
+    for (int k = 0; k < tss::kPairKernels; ++k) {
+        o->block[k] = t[k].block;
+        o->depth[k] = t[k].depth;
+        o->xcd_order[k] = t[k].swz;
+        o->grid[k] = t[k].grid;
+    }
+}
+
+static void tune_from_abi(const tss_tuning* o, tss::PairTune* t) {
+    for (int k = 0; k < tss::kPairKernels; ++k)
+        t[k] = {o->block[k], o->depth[k], o->xcd_order[k], o->grid[k]};
+}
+
+int tss_batch_get_tuning(tss_batch* b, tss_tuning* out) {
     return guarded([&] {
-        require(b, "null argument");
-        auto ok = [](int x) { return x == 64 || x == 128 || x == 256 || x == 512 || x == 1024; };
-        require(ok(dens_block) && ok(force_block), "block sizes must be 64..1024 powers of two");
-        b->dens_block = dens_block;
-        b->force_block = force_block;
-        b->update_shape();
+        require(b && out, "null argument");
+        tune_to_abi(b->tune, out);
+    });
+}
+
+int tss_batch_set_tuning(tss_batch* b, const tss_tuning* t) {
+    return guarded([&] {
+        require(b && t, "null argument");
+        tss::PairTune pt[tss::kPairKernels];
+        tune_from_abi(t, pt);
+        b->set_tuning(pt);
+    });
+}
+
+int tss_batch_autotune(tss_batch* b, int steps_per_trial, int rounds, tss_tuning* out,
+                       int* steps_taken, float* trial_log, int log_rows, int* n_log) {
+    return guarded([&] {
+        require(b && (trial_log || log_rows == 0), "null argument");
+        int taken = b->autotune(steps_per_trial, rounds, trial_log, log_rows, n_log);
+        if (out) tune_to_abi(b->tune, out);
+        if (steps_taken) *steps_taken = taken;
+    });
+}
+
+int tss_get_tuning(tss_solver* s, tss_tuning* out) {
+    return guarded([&] {
+        require(s && out, "null argument");
+        tune_to_abi(s->b.tune, out);
+    });
+}
+
+int tss_set_tuning(tss_solver* s, const tss_tuning* t) {
+    return guarded([&] {
+        require(s && t, "null argument");
+        tss::PairTune pt[tss::kPairKernels];
+        tune_from_abi(t, pt);
+        s->b.set_tuning(pt);
+    });
+}
+
+int tss_autotune(tss_solver* s, int steps_per_trial, int rounds, tss_tuning* out,
+                 int* steps_taken, float* trial_log, int log_rows, int* n_log) {
+    return guarded([&] {
+        require(s && (trial_log || log_rows == 0), "null argument");
+        int taken = s->b.autotune(steps_per_trial, rounds, trial_log, log_rows, n_log);
+        if (out) tune_to_abi(s->b.tune, out);
+        if (steps_taken) *steps_taken = taken;
     });
 }
 
 void tss_batch_destroy(tss_batch* b) { delete b; }
+
+// ---------------------------------------------------------------------------
+// Serving loop: continuous batching of scene requests on one batch
+// ---------------------------------------------------------------------------
+namespace {
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = q * (double)(v.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - (double)i;
+    return i + 1 < v.size() ? v[i] * (1.0 - f) + v[i + 1] * f : v[i];
+}
+
+long long scene_bytes(const tss_config& c, int cap_low, int n_bnd) {
+    tss_batch_desc ds{};
+    ds.max_scenes = 1;
+    ds.cap_low = cap_low;
+    ds.cap_bnd = n_bnd;
+    ds.cells_low = (long long)c.domain_dims[0] * c.domain_dims[1];
+    ds.cells_high = (long long)c.domain_dims_high[0] * c.domain_dims_high[1];
+    return tss_batch::bytes_for(ds);
+}
+
+struct Resident {
+    int req;         // request index
+    int sid;         // scene slot in the batch
+    int left;        // steps still to take
+    long long bytes; // budget accounting
+};
+
+}  // namespace
+
+long long tss_scene_bytes(const tss_config* cfg, int cap_low, int n_bnd) {
+    if (!cfg || cap_low < 1 || n_bnd < 0) return -1;
+    return scene_bytes(*cfg, cap_low, n_bnd);
+}
+
+int tss_serve(const tss_batch_desc* cap, long long budget, tss_request* req, int n,
+              tss_serve_stats* st) {
+    return guarded([&] {
+        require(cap && st && (req || n == 0) && n >= 0 && budget > 0, "invalid argument");
+        std::memset(st, 0, sizeof(*st));
+        auto b = std::make_unique<tss_batch>();
+        b->create(*cap);
+        st->batch_bytes = tss_batch::bytes_for(*cap);
+        st->budget_bytes = budget;
+        std::vector<int> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return req[x].arrival_s < req[y].arrival_s; });
+        int done = 0;
+        for (int i = 0; i < n; ++i) {
+            tss_request& q = req[i];
+            q.status = TSS_E_STATE;  // pending
+            q.admit_s = q.finish_s = -1.0;
+            q.particle_steps = 0;
+            if (q.steps < 1 || !q.cfg || !q.low_xy) {
+                q.status = TSS_E_INVALID;
+                ++done;
+            }
+        }
+        std::vector<Resident> res;
+        std::deque<int> queue;  // arrived, waiting for room
+        std::vector<double> step_ms;
+        size_t next = 0;
+        long long admitted = 0;
+        int stable = 0;  // ticks since the resident set last changed
+        double busy = 0.0, last_finish = 0.0;
+        const auto t0 = std::chrono::steady_clock::now();
+        auto now = [&] {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        };
+        while (done < n) {
+            const double t = now();
+            while (next < order.size() && req[order[next]].arrival_s <= t) {
+                const int r = order[next++];
+                if (req[r].status == TSS_E_STATE) queue.push_back(r);
+            }
+            // admission, in arrival order; a request that does not fit waits
+            bool changed = false;
+            for (auto it = queue.begin(); it != queue.end();) {
+                tss_request& q = req[*it];
+                const long long sb = scene_bytes(*q.cfg, q.cap_low, q.n_bnd);
+                if (sb > budget || sb > st->batch_bytes) {  // can never fit
+                    q.status = TSS_E_NOMEM;
+                    q.finish_s = now();
+                    ++done;
+                    it = queue.erase(it);
+                    continue;
+                }
+                if (admitted + sb > budget) {
+                    ++it;
+                    continue;
+                }
+                int sid = -1;
+                try {
+                    sid = b->add_scene(*q.cfg, q.low_xy, q.n_low, q.cap_low, q.bnd_xy, q.n_bnd);
+                } catch (const TssError& e) {
+                    if (e.code == TSS_E_NOMEM && !res.empty()) {  // wait for room
+                        ++it;
+                        continue;
+                    }
+                    q.status = e.code;
+                    q.finish_s = now();
+                    ++done;
+                    it = queue.erase(it);
+                    continue;
+                }
+                q.admit_s = now();
+                admitted += sb;
+                res.push_back({*it, sid, q.steps, sb});
+                it = queue.erase(it);
+                changed = true;
+            }
+            if (res.empty()) {
+                if (next >= order.size()) {
+                    // nothing resident, nothing to come: the queue cannot
+                    // be admitted (it was rejected above) -- done
+                    if (queue.empty()) break;
+                    for (int r : queue) {
+                        req[r].status = TSS_E_NOMEM;
+                        req[r].finish_s = now();
+                        ++done;
+                    }
+                    queue.clear();
+                    break;
+                }
+                // idle until the next arrival
+                const double wait = req[order[next]].arrival_s - now();
+                if (wait > 0)
+                    std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
+                continue;
+            }
+            st->max_resident = std::max(st->max_resident, (int)res.size());
+            st->peak_admitted_bytes = std::max(st->peak_admitted_bytes, admitted);
+            if (changed) stable = 0;
+            int minleft = res[0].left;
+            for (auto& r : res) minleft = std::min(minleft, r.left);
+            // graph pairs once the resident set has held for two ticks
+            const int k = (stable >= 2 && minleft >= 2) ? 2 : 1;
+            const double a = now();
+            if (k == 2) {
+                b->step_graph(2);
+                ++st->graph_ticks;
+            } else {
+                b->step_eager(1);
+            }
+            HIP_CHECK(hipStreamSynchronize(b->stream));
+            const double e = now();
+            busy += e - a;
+            for (int i = 0; i < k; ++i) step_ms.push_back((e - a) * 1e3 / k);
+            ++st->ticks;
+            ++stable;
+            bool removed = false;
+            for (auto it = res.begin(); it != res.end();) {
+                it->left -= k;
+                if (it->left > 0) {
+                    ++it;
+                    continue;
+                }
+                tss_request& q = req[it->req];
+                for (int f = 0; f < TSS_FIELD_COUNT; ++f)
+                    if (q.out[f]) b->read_field(it->sid, f, q.out[f], q.out_bytes[f]);
+                tss_counts_t c;
+                b->counts(it->sid, &c);
+                q.particle_steps = c.particle_steps;
+                q.finish_s = e;
+                q.status = TSS_OK;
+                st->particle_steps += c.particle_steps;
+                last_finish = e;
+                b->remove_scene(it->sid);
+                admitted -= it->bytes;
+                ++done;
+                removed = true;
+                it = res.erase(it);
+            }
+            if (removed) stable = 0;
+        }
+        st->wall_s = last_finish;
+        st->busy_s = busy;
+        st->particle_steps_per_s = last_finish > 0 ? (double)st->particle_steps / last_finish : 0.0;
+        std::vector<double> rl, ql;
+        for (int i = 0; i < n; ++i) {
+            if (req[i].status != TSS_OK) continue;
+            rl.push_back((req[i].finish_s - req[i].arrival_s) * 1e3);
+            ql.push_back((req[i].admit_s - req[i].arrival_s) * 1e3);
+        }
+        st->step_ms_p50 = percentile(step_ms, 0.50);
+        st->step_ms_p99 = percentile(step_ms, 0.99);
+        st->request_ms_p50 = percentile(rl, 0.50);
+        st->request_ms_p99 = percentile(rl, 0.99);
+        st->queue_ms_p50 = percentile(ql, 0.50);
+        st->queue_ms_p99 = percentile(ql, 0.99);
+    });
+}
 
 int tss_bandwidth_probe(size_t bytes, int reps, double* gbs) {
     return guarded([&] {

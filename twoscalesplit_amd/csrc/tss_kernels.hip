@@ -925,6 +925,7 @@ __global__ __launch_bounds__(256) void k_finalize(DevState d) {
         for (int f = 0; f < kRedFields; ++f) tot[f] = sh[f][0];
         c.n_l_last = c.seg_l1 - c.seg_l0;
         c.n_h_last = c.seg_h1 - c.seg_h0;
+        c.psteps += (unsigned long long)(c.n_l_last + c.n_h_last);
         // next lists: low = kept + merged; high = kept + 4 per split
         c.n_split = c.sp1 - c.sp0;
         c.n_l = (c.kl1 - c.kl0) + c.n_merge;
@@ -1227,15 +1228,13 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     const int cur = enqueue_grid(d, sh, st);
     const uint32_t* vals = d.vals[cur];
     mark(1);
-    const int g_low = blocks_for(d.CL, kPairBlock, 1 << 30);
-    const int g_high = blocks_for(4LL * d.CL, kPairBlock, 1 << 30);
-    k_density<false><<<g_low, kPairBlock, 0, st>>>(d, vals);
+    launch_pair<false, false>(d, vals, sh.tune[kTuneDensityLow], d.CL, st);
     mark(2);
-    k_density<true><<<g_high, kPairBlock, 0, st>>>(d, vals);
+    launch_pair<false, true>(d, vals, sh.tune[kTuneDensityHigh], 4LL * d.CL, st);
     mark(3);
-    k_force<false><<<g_low, kPairBlock, 0, st>>>(d, vals);
+    launch_pair<true, false>(d, vals, sh.tune[kTuneForceLow], d.CL, st);
     mark(4);
-    k_force<true><<<g_high, kPairBlock, 0, st>>>(d, vals);
+    launch_pair<true, true>(d, vals, sh.tune[kTuneForceHigh], 4LL * d.CL, st);
     mark(5);
     k_reduce<<<sh.red_slots, 256, 0, st>>>(d, vals);
     mark(6);

@@ -7,7 +7,8 @@ C3 large      : 1414x707 low (dx = 0.8/707) in [0,4]x[0,1], CFL 0.4;
                 ``forced=True`` splits every low particle at the first step
                 (999,698 low + 3,998,792 high)
 C4 batched    : 256 scenes, low counts LogU[2k, 64k], aspect U(0.25, 2),
-                alpha U(0.02, 0.1), cap 4x low
+                alpha U(0.02, 0.1), cap 4x low; band rules, or the reference's
+                split rule at mid-block (sustained two-scale load)
 C5 served     : requests with low counts LogU[4k, 256k], steps U[500, 5000]
 
 Kernel support radius h = 4 dx ("support 2h, h = 2dx" of the smoothing
@@ -156,14 +157,26 @@ def _scene_from_count(seed: int, idx: int, n_low: int, aspect: float,
 
 
 def c4_batched(seed: int = SEED, n_scenes: int = 256, lo: int = 2000,
-               hi: int = 64000, steps: int = 300) -> list[Scene]:
+               hi: int = 64000, steps: int = 300,
+               rule: str = "band") -> list[Scene]:
+    """rule "band": the surface-band injection/removal rules; "reference":
+    the reference's split rule (split once x > split_x, never merged) with
+    split_x at the middle of each fluid block, so half of every scene runs
+    two-scale from the first step on."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for i in range(n_scenes):
         n = int(math.exp(rng.uniform(math.log(lo), math.log(hi))))
         aspect = float(rng.uniform(0.25, 2.0))
         alpha = float(rng.uniform(0.02, 0.1))
-        out.append(_scene_from_count(seed, i, n, aspect, alpha, steps))
+        sc = _scene_from_count(seed, i, n, aspect, alpha, steps)
+        if rule == "reference":
+            sc.cfg.mode = N.MODE_REFERENCE
+            nx = int(round(sc.cfg.domain_end[0] / 2.5 / sc.meta["dx"]))
+            sc.cfg.split_x = float(np.float32(0.5 * nx * sc.meta["dx"]))
+        elif rule != "band":
+            raise ValueError(f"unknown rule {rule!r}")
+        out.append(sc)
     return out
 
 

@@ -171,6 +171,27 @@ class Solver:
                 "tss_work_counts")
         return {k: int(v) for k, v in zip(WORK, out)}
 
+    # -- launch shapes -------------------------------------------------------
+    def get_tuning(self) -> N.TssTuning:
+        t = N.TssTuning()
+        N.check(N.lib().tss_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return t
+
+    def set_tuning(self, t: N.TssTuning) -> None:
+        N.check(N.lib().tss_set_tuning(self._h, C.byref(t)), "set_tuning")
+
+    def autotune(self, steps_per_trial: int = 1, rounds: int = 2) -> dict:
+        """Native autotuner (advances the scene by real steps)."""
+        t = N.TssTuning()
+        taken = C.c_int()
+        log = np.zeros((64, 8), np.float32)
+        nlog = C.c_int()
+        N.check(N.lib().tss_autotune(self._h, steps_per_trial, rounds,
+                                   C.byref(t), C.byref(taken), N.ptr(log), 64,
+                                   C.byref(nlog)), "autotune")
+        return {"tuning": t.as_dict(), "steps_taken": taken.value,
+                "trials": log[:nlog.value].tolist(), "_struct": t}
+
     def snapshot(self) -> dict:
         """Every by-id field, the lists and dt (synchronising)."""
         cnt = self.counts()
@@ -279,7 +300,25 @@ class Batch:
                 "tss_batch_num_scenes")
         return n.value
 
-    def set_tuning(self, density_block: int, force_block: int) -> None:
-        N.check(N.lib().tss_batch_set_tuning(self._h, density_block,
-                                             force_block),
+    def get_tuning(self) -> N.TssTuning:
+        t = N.TssTuning()
+        N.check(N.lib().tss_batch_get_tuning(self._h, C.byref(t)),
+                "tss_batch_get_tuning")
+        return t
+
+    def set_tuning(self, t: N.TssTuning) -> None:
+        N.check(N.lib().tss_batch_set_tuning(self._h, C.byref(t)),
                 "tss_batch_set_tuning")
+
+    def autotune(self, steps_per_trial: int = 1, rounds: int = 2) -> dict:
+        """Native autotuner (advances every scene by real steps)."""
+        t = N.TssTuning()
+        taken = C.c_int()
+        log = np.zeros((64, 8), np.float32)
+        nlog = C.c_int()
+        N.check(N.lib().tss_batch_autotune(self._h, steps_per_trial, rounds,
+                                           C.byref(t), C.byref(taken),
+                                           N.ptr(log), 64, C.byref(nlog)),
+                "tss_batch_autotune")
+        return {"tuning": t.as_dict(), "steps_taken": taken.value,
+                "trials": log[:nlog.value].tolist(), "_struct": t}
