@@ -22,7 +22,6 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
-import math
 import sys
 import time
 
@@ -36,15 +35,17 @@ BUDGET_BYTES = 200 * 10**9
 
 
 class Trace:
-    """Requests (scenes + arrival times) of one Poisson stream."""
+    """Requests (scenes + arrival times) of one Poisson stream. The scenes
+    depend on the seed only, the arrival times on the seed and the rate."""
 
     def __init__(self, seed: int, rate: float, n: int, lo: int = 4000,
-                 hi: int = 256000, smin: int = 500, smax: int = 5000):
+                 hi: int = 256000, smin: int = 500, smax: int = 5000,
+                 scene_list=None):
         rng = np.random.Generator(np.random.PCG64([seed, int(rate * 1000)]))
         gaps = rng.exponential(1.0 / rate, n)
         self.arrival = np.cumsum(gaps) - gaps[0]
-        self.scenes = [scenes.c5_request(seed, i, lo, hi, smin, smax)
-                       for i in range(n)]
+        self.scenes = scene_list if scene_list is not None else [
+            scenes.c5_request(seed, i, lo, hi, smin, smax) for i in range(n)]
         self.rate = rate
 
 
@@ -112,10 +113,11 @@ def _out_array(name: str, s) -> np.ndarray:
     return np.zeros(shapes[name], dt)
 
 
-def scene_at_a_time(trace: Trace) -> dict:
-    """Each request alone on a fresh solver, FIFO against the arrivals."""
+def service_times(scene_list) -> tuple[list[float], list[int]]:
+    """Each request alone on a fresh single-scene solver (graph-replayed
+    steps): its service time and particle-steps."""
     service, psteps = [], []
-    for s in trace.scenes:
+    for s in scene_list:
         sol = Solver(s.cfg, s.low_xy, s.cap_low, s.bnd_xy)
         sol.synchronize()
         a = time.perf_counter()
@@ -124,6 +126,13 @@ def scene_at_a_time(trace: Trace) -> dict:
         service.append(time.perf_counter() - a)
         psteps.append(sol.counts()["particle_steps"])
         sol.close()
+    return service, psteps
+
+
+def scene_at_a_time(trace: Trace, measured=None) -> dict:
+    """Requests served one at a time, first-in first-out, against the
+    trace's arrivals (service times measured alone)."""
+    service, psteps = measured or service_times(trace.scenes)
     done, lat = 0.0, []
     for a, sv in zip(trace.arrival, service):
         done = max(done, a) + sv
@@ -137,8 +146,9 @@ def scene_at_a_time(trace: Trace) -> dict:
 
 
 def run_rate(seed: int, rate: float, n: int, pool_low: int,
-             baseline: bool = True, budget: int = BUDGET_BYTES) -> dict:
-    tr = Trace(seed, rate, n)
+             baseline: bool = True, budget: int = BUDGET_BYTES,
+             scene_list=None, measured=None) -> dict:
+    tr = Trace(seed, rate, n, scene_list=scene_list)
     desc = pool_desc(tr, pool_low)
     st, per, _ = serve(tr, desc, budget)
     ok = sum(1 for p in per if p["status"] == 0)
@@ -159,7 +169,7 @@ def run_rate(seed: int, rate: float, n: int, pool_low: int,
         "pool_low_capacity": desc.cap_low,
     }
     if baseline:
-        b = scene_at_a_time(tr)
+        b = scene_at_a_time(tr, measured)
         out["scene_at_a_time"] = {k: (round(v, 3) if isinstance(v, float)
                                       else v) for k, v in b.items()}
         out["speedup_vs_scene_at_a_time"] = round(
@@ -178,11 +188,18 @@ def main(argv=None) -> int:
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-baseline", action="store_true")
     args = ap.parse_args(argv)
+    scene_list = Trace(args.seed, 1.0, args.requests).scenes
+    measured = None
+    if not args.no_baseline:
+        print("[serve] scene-at-a-time service times", file=sys.stderr,
+              flush=True)
+        measured = service_times(scene_list)
     for r in args.rates:
         print(f"[serve] rate {r}/s, {args.requests} requests", file=sys.stderr,
               flush=True)
         res = run_rate(args.seed, r, args.requests, args.pool_low,
-                       not args.no_baseline, int(args.budget_gb * 1e9))
+                       not args.no_baseline, int(args.budget_gb * 1e9),
+                       scene_list, measured)
         print(json.dumps(res), flush=True)
     return 0
 
