@@ -65,6 +65,24 @@ def test_reference_rule_parity():
     assert sum(r["events"][0] for r in rep) > 0
 
 
+def test_crowded_targets_parity():
+    """Half of the block squeezed to 0.6 dx spacing: up to ~130 same-level
+    neighbours within h there and ~50 in the other half, in the same waves
+    (long and short candidate runs, full LDS buffers); split to populate
+    the high level."""
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_REFERENCE,
+                                split_x=0.02)
+    xy = sc.low_xy.astype(np.float64)
+    left = xy[:, 0] < np.median(xy[:, 0])
+    x0, y0 = xy[left, 0].min(), xy[left, 1].min()
+    xy[left, 0] = x0 + 0.6 * (xy[left, 0] - x0)
+    xy[left, 1] = y0 + 0.6 * (xy[left, 1] - y0)
+    sc.low_xy = xy.astype(np.float32)
+    rep = []
+    _run_checks(sc, [0, 0, 0], rep)
+    print(json.dumps(rep))
+
+
 def test_promote_merge_dissolve_parity():
     sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND)
     sc.cfg.promote_min_age = 1
