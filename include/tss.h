@@ -267,13 +267,14 @@ int tss_batch_num_scenes(tss_batch* b, int* n);
 int tss_batch_get_tuning(tss_batch* b, tss_tuning* out);
 int tss_batch_set_tuning(tss_batch* b, const tss_tuning* t);
 /* Autotuner: times candidate shapes with hipEvents while advancing the
- * batch by real steps (they count, the results being shape-independent):
- * stage 1 tries every block x depth, stage 2 the XCD order and persistent
- * grid caps (4 and 8 blocks per CU) on each kernel's stage-1 winner; each
- * candidate runs `rounds` times for steps_per_trial steps, interleaved, and
- * the fastest total per kernel is kept (and returned in out). steps_taken:
- * steps advanced. trial_log (optional): log_rows x 8 floats per trial,
- * (block, depth, xcd_order, grid, ms of kernels 0..3); n_log rows used. */
+ * batch by real steps (they count, the results being shape-independent).
+ * After one ordinary step, stage 1 tries every block x depth, stage 2 the
+ * XCD order and persistent grid caps (4 and 8 blocks per CU) on each
+ * kernel's stage-1 winner; each candidate runs `rounds` times for
+ * steps_per_trial steps, in alternating order, and per kernel the lowest
+ * mean time per target of its level is kept (and returned in out).
+ * steps_taken: steps advanced. trial_log (optional): log_rows x 8 floats per
+ * trial, (block, depth, xcd_order, grid, ms of kernels 0..3); n_log rows. */
 int tss_batch_autotune(tss_batch* b, int steps_per_trial, int rounds,
                        tss_tuning* out, int* steps_taken, float* trial_log,
                        int log_rows, int* n_log);

@@ -85,8 +85,8 @@ def test_autotune_steps_are_ordinary_steps():
         ib = [b.add_scene(s.cfg, s.low_xy, s.cap_low, s.bnd_xy) for s in scs]
         res = a.autotune(steps_per_trial=1, rounds=1)
         taken = res["steps_taken"]
-        assert taken == 9 + 5
-        assert len(res["trials"]) == taken
+        assert taken == 1 + 9 + 5
+        assert len(res["trials"]) == taken - 1
         assert all(row[4 + k] > 0 for row in res["trials"] for k in range(4))
         a.step_graph(4)
         b.step(taken)

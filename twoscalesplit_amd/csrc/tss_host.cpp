@@ -250,6 +250,8 @@ struct tss_batch {
                 "invalid batch capacities");
         require(5 * ds.cap_low < (1LL << 31) && ds.cells_low + ds.cells_high < (1LL << 31),
                 "batch capacities exceed 32-bit indexing");
+        // neighbour gathers use 32-bit byte offsets into the sorted copies
+        require(16 * 5 * ds.cap_low < (1LL << 32), "batch capacities exceed 32-bit byte offsets");
         cap_l = ds.cap_low;
         cap_b = ds.cap_bnd;
         ccl = ds.cells_low;
@@ -817,10 +819,11 @@ struct tss_batch {
     // Project-written autotuner. Each trial applies one candidate shape to
     // all four neighbour kernels, advances the batch `steps` real steps
     // (results do not depend on the shape, so the steps count) and reads
-    // each kernel's mean time from the phase events. Candidates are visited
-    // `rounds` times in interleaved order; the fastest total per kernel wins.
-    // Stage 1: block x depth with the plain grid; stage 2: XCD block order x
-    // persistent grid caps on top of each kernel's stage-1 winner.
+    // each kernel's mean time from the phase events. After one ordinary
+    // step, candidates are visited `rounds` times in alternating order; the
+    // lowest mean time per target wins, per kernel. Stage 1: block x depth
+    // with the plain grid; stage 2: XCD block order x persistent grid caps on
+    // top of each kernel's stage-1 winner.
     int autotune(int steps, int rounds, float* trial_log, int log_cap, int* n_log) {
         require(steps >= 1 && rounds >= 1, "autotune: steps and rounds must be >= 1");
         int alive = 0;
@@ -831,18 +834,32 @@ struct tss_batch {
         HIP_CHECK(hipGetDevice(&dev));
         HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         int taken = 0, logged = 0;
+        // per-target time of each kernel: the population changes from step to
+        // step (the first step of a scene splits before any child exists),
+        // so a trial's kernel time is divided by the targets of its level in
+        // that step; trials whose level had no target do not count
         auto run_trials = [&](const std::vector<std::array<tss::PairTune, tss::kPairKernels>>& cand,
                               std::vector<std::array<double, tss::kPairKernels>>& tot) {
             tot.assign(cand.size(), {0.0, 0.0, 0.0, 0.0});
+            std::vector<std::array<int, tss::kPairKernels>> cnt(cand.size(), {0, 0, 0, 0});
             for (int r = 0; r < rounds; ++r) {
-                for (size_t c = 0; c < cand.size(); ++c) {
+                for (size_t i = 0; i < cand.size(); ++i) {
+                    // alternate the visiting order so drift cancels
+                    const size_t c = (r & 1) ? cand.size() - 1 - i : i;
                     set_tuning(cand[c].data());
                     set_timing(true);
                     step_eager(steps);
                     taken += steps;
                     float ph[kPhases];
                     phase_times(ph);
-                    for (int k = 0; k < tss::kPairKernels; ++k) tot[c][k] += ph[1 + k];
+                    const tss::DevCounts dc = read_counts();
+                    const int ntg[tss::kPairKernels] = {dc.n_l_last, dc.n_h_last, dc.n_l_last,
+                                                        dc.n_h_last};
+                    for (int k = 0; k < tss::kPairKernels; ++k) {
+                        if (ntg[k] <= 0) continue;
+                        tot[c][k] += (double)ph[1 + k] / (double)ntg[k];
+                        ++cnt[c][k];
+                    }
                     if (trial_log && logged < log_cap) {
                         float* row = trial_log + (size_t)logged * 8;
                         const auto& t = cand[c][0];
@@ -855,7 +872,14 @@ struct tss_batch {
                     }
                 }
             }
+            for (size_t c = 0; c < cand.size(); ++c)
+                for (int k = 0; k < tss::kPairKernels; ++k)
+                    tot[c][k] = cnt[c][k] ? tot[c][k] / cnt[c][k] : 1e300;
         };
+        // one ordinary step first: trials then start from a populated state
+        set_timing(false);
+        step_eager(1);
+        taken += 1;
         // stage 1
         std::vector<std::array<tss::PairTune, tss::kPairKernels>> cand;
         for (int b : {128, 256, 512})

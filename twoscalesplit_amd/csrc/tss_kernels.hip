@@ -49,6 +49,21 @@ __device__ __forceinline__ float dist2(float2 a, float2 b) {
     return p + q;
 }
 
+// Gather of element i of a device array through a 32-bit byte offset: with
+// the base in scalar registers this is one 32-bit shift per load instead of
+// 64-bit address arithmetic (batches keep every such array below 4 GiB).
+template <class T>
+__device__ __forceinline__ T gather(const T* __restrict__ base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       (size_t)(i * (uint32_t)sizeof(T)));
+}
+
+// 0 < r2 <= t as one unsigned comparison of the bit patterns (r2 is a sum
+// of squares, so never negative; NaN compares above every finite t).
+__device__ __forceinline__ bool in_open_ball(float r2, float t) {
+    return __float_as_uint(r2) - 1u < __float_as_uint(t);
+}
+
 __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
     float y = x - c;
     float t = s + y;
