@@ -83,6 +83,46 @@ def test_crowded_targets_parity():
     print(json.dumps(rep))
 
 
+def _four_span(o, R, dims, near):
+    """A float32 coordinate whose +-R box spans four cells of size R."""
+    from test_oracle_pins import _boundary_positions, _spans
+    y = _boundary_positions(np.float32(o), np.float32(R), dims)
+    n = _spans(y, np.float32(R), np.float32(o), dims)
+    c = y[n == 4]
+    return np.float32(c[np.argmin(np.abs(c - near))])
+
+
+def test_four_row_box_parity():
+    """Targets whose scan box spans 4 x 4 cells (FP32 rounding at two cell
+    edges), on both levels: four rows of four cells are scanned."""
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_REFERENCE,
+                                split_x=0.08)
+    cfg = sc.cfg
+    h, H = cfg.effective_radius, cfg.effective_radius_high
+    xl = _four_span(cfg.domain_origin[0], h, cfg.domain_dims[0], 0.15)
+    yl = _four_span(cfg.domain_origin[1], h, cfg.domain_dims[1], 0.15)
+    low = sc.low_xy.copy()
+    k = int(np.argmin(np.hypot(low[:, 0] - xl, low[:, 1] - yl)))
+    low[k] = (xl, yl)
+    sc.low_xy = low
+    _say("four-row boxes: low particle", k, "at", float(xl), float(yl))
+    sol = _solver(sc)
+    try:
+        check_step(sol, cfg, sc.bnd_xy, [])
+        sol.step(1)
+        xh = _four_span(cfg.domain_origin[0], H, cfg.domain_dims_high[0], 0.1)
+        yh = _four_span(cfg.domain_origin[1], H, cfg.domain_dims_high[1], 0.1)
+        lh = sol.read("LIST_H")
+        ph = sol.read("POS_H")
+        a = int(lh[np.argmin(np.hypot(ph[lh, 0] - xh, ph[lh, 1] - yh))])
+        ph[a] = (xh, yh)
+        sol.write("POS_H", ph)
+        _say("four-row boxes: high particle", a, "at", float(xh), float(yh))
+        check_step(sol, cfg, sc.bnd_xy, [])
+    finally:
+        sol.close()
+
+
 def test_promote_merge_dissolve_parity():
     sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND)
     sc.cfg.promote_min_age = 1

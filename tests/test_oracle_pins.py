@@ -476,3 +476,40 @@ def test_merge_places_parent_at_centre_of_mass():
         np.testing.assert_allclose(m_l * a["post.VEL_L"][p].astype(np.float64),
                                    m_h * cv.sum(axis=0), rtol=1e-6, atol=1e-9)
         assert p in listed and a["post.STATE_L"][p] == 0
+
+
+def _boundary_positions(o, R, dims):
+    """Every float32 within 200 ulps of the cell edges that x - R, x and
+    x + R can meet on a grid of cell R."""
+    f32 = np.float32
+    ks = np.arange(0, dims + 2)
+    c = np.concatenate([o + ks * R - R, o + ks * R, o + ks * R + R]).astype(f32)
+    out, up, dn = [c], c.copy(), c.copy()
+    for _ in range(200):
+        up = np.nextafter(up, f32(np.inf))
+        dn = np.nextafter(dn, f32(-np.inf))
+        out += [up, dn]
+    return np.concatenate(out)
+
+
+def _spans(y, R, o, dims):
+    lo = np.array([L.orc_axis_cell(float(v), -float(R), float(o), float(R), int(dims)) for v in y])
+    hi = np.array([L.orc_axis_cell(float(v), float(R), float(o), float(R), int(dims)) for v in y])
+    return hi - lo + 1
+
+
+def test_scan_box_spans_at_most_four_cells():
+    """A box of radius R on a grid of cell R spans 3 cells per axis, 4 where
+    FP32 rounding straddles two edges, never 5 (a row run covers at most
+    four cells)."""
+    from twoscalesplit_amd import scenes
+    cfg = scenes.c2_two_scale().cfg
+    f32 = np.float32
+    seen4 = False
+    for R, dims in ((f32(cfg.effective_radius), cfg.domain_dims[1]),
+                    (f32(cfg.effective_radius_high), cfg.domain_dims_high[1])):
+        y = _boundary_positions(f32(cfg.domain_origin[1]), R, dims)
+        n = _spans(y, R, f32(cfg.domain_origin[1]), dims)
+        assert n.max() <= 4
+        seen4 = seen4 or (n == 4).any()
+    assert seen4
