@@ -84,6 +84,14 @@ typedef struct tss_config {
     int cap_high;             /* max live high particles (<= 4*cap_low)     */
     int dt_mode;              /* TSS_DT_*                                  */
     float cfl, dt_max;
+    /* pressure-only relaxation of the children injected by the previous
+     * step's splits, run after the grid build: relax_iters Jacobi
+     * iterations (0 = off) of density + pressure/wall acceleration +
+     * symplectic Euler at relax_dt_frac * dt (the dormant sub-steps of
+     * TwoScale2D-master/TwoScale2D/WCSPHSolver.cu:2380-2446 and
+     * R/WCSPHSolver.cu:3044-3110, there 4 iterations at dt/10) */
+    int relax_iters;
+    float relax_dt_frac;
 } tss_config;
 
 typedef struct tss_solver tss_solver;
@@ -193,7 +201,8 @@ int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums,
 /* Device time per phase (ms), averaged over every step taken (eager or
  * graph-replayed) since timing was switched on, from hipEvents recorded on
  * the solver's stream around each phase without a host wait per step:
- * [0] grid build (hash, radix sort, cell ranges, reorder) [1] density, low
+ * [0] grid build (hash, radix sort, cell ranges, reorder; and the
+ * relaxation of new children when a scene enables it) [1] density, low
  * targets [2] density, high targets [3] force + integration, low targets
  * [4] force + integration, high targets [5] FP64 reductions [6] adaptivity
  * and list compaction [7] finalize (counts, dt). tss_enable_timing resets

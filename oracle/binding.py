@@ -33,6 +33,7 @@ class OrcParams(C.Structure):
         ("promote_min_age", C.c_int), ("promote_max_age", C.c_int),
         ("promote_fluc", C.c_float), ("cap_h", C.c_int),
         ("dt_mode", C.c_int), ("cfl", C.c_float), ("dt_max", C.c_float),
+        ("relax_iters", C.c_int), ("relax_frac", C.c_float),
     ]
 
 
@@ -45,7 +46,7 @@ class OrcState(C.Structure):
         ("p_h", _P), ("state_h", _P), ("pos_b", _P),
         ("n_l", C.c_int), ("n_h", C.c_int), ("n_g", C.c_int),
         ("list_l", _P), ("list_h", _P), ("list_g", _P),
-        ("dt", C.c_float), ("step", C.c_int),
+        ("dt", C.c_float), ("step", C.c_int), ("n_fresh", C.c_int),
     ]
 
 
@@ -99,6 +100,8 @@ def lib() -> C.CDLL:
             "orc_v2max": (C.c_double, [SS]),
             "orc_conservation": (None, [PS, SS, _P, _P]),
             "orc_step": (C.c_int, [PS, SS, C.c_int, C.c_int]),
+            "orc_relax_f64": (None, [PS, SS, GS]),
+            "orc_relax_f32": (None, [PS, SS, GS]),
             "orc_kernel_w": (C.c_double, [C.c_double, C.c_double]),
             "orc_kernel_gradf": (C.c_double, [C.c_double, C.c_double]),
             "orc_wall_gamma": (C.c_double, [C.c_double, C.c_double,
@@ -138,6 +141,8 @@ def params_from_config(cfg) -> OrcParams:
     p.promote_fluc = cfg.promote_fluc
     p.cap_h = cfg.cap_high
     p.dt_mode, p.cfl, p.dt_max = cfg.dt_mode, cfg.cfl, cfg.dt_max
+    p.relax_iters = getattr(cfg, "relax_iters", 0)
+    p.relax_frac = getattr(cfg, "relax_dt_frac", 0.1)
     return p
 
 
@@ -197,6 +202,8 @@ class Scene:
         self.s.n_l, self.s.n_h, self.s.n_g = (c["n_low"], c["n_high"],
                                               c["n_groups"])
         self.s.dt = float(snap["DT"][0])
+        # children of the last step's splits close the high list
+        self.s.n_fresh = 4 * int(c.get("n_split", 0))
 
     @property
     def n(self) -> tuple[int, int, int]:
@@ -292,6 +299,11 @@ class Scene:
         lib().orc_conservation(C.byref(self.p), C.byref(self.s), ptr(out),
                                ptr(ab))
         return out, ab
+
+    def relax(self, g: dict, precision: int = 64) -> None:
+        """Relaxation of the last n_fresh children of list_h on grid g."""
+        fn = lib().orc_relax_f64 if precision == 64 else lib().orc_relax_f32
+        fn(C.byref(self.p), C.byref(self.s), C.byref(g["_struct"]))
 
     def step(self, precision: int = 64, nthreads: int = 0) -> None:
         rc = lib().orc_step(C.byref(self.p), C.byref(self.s), precision,

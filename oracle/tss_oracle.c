@@ -58,6 +58,8 @@ int orc_params_init(orc_params* p, float xs, float ys, float xe, float ye,
     p->dt_mode = ORC_DT_FIXED;
     p->cfl = 0.4f;
     p->dt_max = 1e-3f;
+    p->relax_iters = 0;
+    p->relax_frac = 0.1f;
     return 0;
 }
 
@@ -609,6 +611,7 @@ void orc_adapt(const orc_params* p, orc_state* s, const orc_grid* g,
     s->n_l = nl;
     s->n_h = nh;
     s->n_g = ng;
+    s->n_fresh = 4 * out->n_split;  /* appended last: the tail of list_h */
     free(merged);
 }
 
@@ -732,6 +735,13 @@ int orc_step(const orc_params* p, orc_state* s, int precision, int nthreads)
 
     orc_build_boundary_grid(p, s, &g);
     orc_build_grid(p, s, &g);
+    if (p->relax_iters > 0 && s->n_fresh > 0) {
+        if (precision == 64) {
+            orc_relax_f64(p, s, &g);
+        } else {
+            orc_relax_f32(p, s, &g);
+        }
+    }
     if (precision == 64) {
         double* rl = (double*)malloc(sizeof(double) * (size_t)cap_l);
         double* pl = (double*)malloc(sizeof(double) * (size_t)cap_l);

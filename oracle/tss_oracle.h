@@ -69,6 +69,8 @@ typedef struct orc_params {
     int   cap_h;                  /* live high-particle cap (<= 4*cap_l)        */
     int   dt_mode;
     float cfl, dt_max;
+    int   relax_iters;            /* relaxation of new children (0: off)        */
+    float relax_frac;             /* relaxation step = relax_frac * dt          */
 } orc_params;
 
 /* Scene state, by particle id. High id of child k of low parent p is 4p+k.
@@ -87,6 +89,8 @@ typedef struct orc_state {
     int32_t *list_l, *list_h, *list_g; /* cap_l, 4*cap_l, cap_l */
     float dt;
     int step;
+    int n_fresh;                       /* children injected by the last step:
+                                          the last n_fresh of list_h       */
 } orc_state;
 
 /* Neighbour grid of one step (outputs of the grid phase). */
@@ -185,8 +189,15 @@ double orc_tait(double rho, double rho0, double B);
 void orc_conservation(const orc_params* p, const orc_state* s, double* out,
                       double* abs_sum);
 
-/* One full step: grid, density, force, integrate, adapt, dt. precision 64
- * or 32 selects the physics build. nthreads <= 0 keeps OpenMP's default. */
+/* Relaxation of the s->n_fresh children at the end of list_h on the grid g
+ * (p->relax_iters Jacobi iterations of density, pressure + wall
+ * acceleration and symplectic Euler at relax_frac * dt; see orc_step). */
+void orc_relax_f64(const orc_params* p, orc_state* s, const orc_grid* g);
+void orc_relax_f32(const orc_params* p, orc_state* s, const orc_grid* g);
+
+/* One full step: grid, [relaxation], density, force, integrate, adapt, dt.
+ * precision 64 or 32 selects the physics build. nthreads <= 0 keeps
+ * OpenMP's default. */
 int  orc_step(const orc_params* p, orc_state* s, int precision, int nthreads);
 
 #ifdef __cplusplus

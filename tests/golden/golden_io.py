@@ -35,9 +35,13 @@ def config_to_dict(cfg) -> dict:
 
 
 def config_from_dict(d: dict):
+    """Fields a fixture predates keep the values of a zeroed config (every
+    optional control off)."""
     from twoscalesplit_amd import _native as N
     cfg = N.TssConfig()
     for name, typ in cfg._fields_:
+        if name not in d:
+            continue
         v = d[name]
         if isinstance(v, list):
             arr = getattr(cfg, name)

@@ -513,3 +513,45 @@ def test_scan_box_spans_at_most_four_cells():
         assert n.max() <= 4
         seen4 = seen4 or (n == 4).any()
     assert seen4
+
+
+def test_relaxation_of_two_children_closed_form():
+    """Two fresh children alone (no walls, no other particles), at rest:
+    one relaxation iteration moves each by dt_r^2 a with
+    a_i = -m_H (2 p / rho^2) grad W_bar(r) (pressure only; W_bar the blended
+    kernel, both densities equal by symmetry), equal and opposite."""
+    from twoscalesplit_amd import MODE_REFERENCE, scenes
+    sc = scenes.small_two_scale(nx=2, ny=2, nxd=64, nyd=64,
+                                mode=MODE_REFERENCE)
+    cfg = sc.cfg
+    cfg.relax_iters = 1
+    cfg.relax_dt_frac = 0.1
+    o = ob.Scene(ob.params_from_config(cfg), 4, np.zeros((0, 2), np.float32))
+    o.s.dt = cfg.time_step
+    h, H = float(np.float32(cfg.effective_radius)), float(np.float32(cfg.effective_radius_high))
+    p0 = np.array([0.2, 0.2], np.float32)
+    p1 = np.array([0.2 + 0.31 * H, 0.2 + 0.17 * H], np.float32)
+    o.a["pos_h"][0], o.a["pos_h"][1] = p0, p1
+    o.a["state_h"][:2] = 1
+    o.a["list_h"][:2] = [0, 1]
+    o.s.n_h = 2
+    o.s.n_fresh = 2
+    g = o.grid()
+    o.relax(g)
+    X = p0.astype(np.float64) - p1.astype(np.float64)
+    r = math.hypot(*X)
+    m_h = float(np.float32(cfg.mass_high))
+    wbar = lambda x: (L.orc_kernel_w(x, h) + L.orc_kernel_w(x, H)) / 2
+    rho = m_h * (wbar(0.0) + wbar(r))
+    assert float(o.a["rho_h"][0]) == pytest.approx(rho, rel=1e-6)
+    p = L.orc_tait(rho, float(np.float32(cfg.rest_density)), float(np.float32(cfg.tait_coefficient)))
+    gbar = (L.orc_kernel_gradf(r, h) + L.orc_kernel_gradf(r, H)) / 2
+    a0 = -m_h * (2 * p / rho ** 2) * gbar * X / r
+    dt_r = float(np.float32(np.float32(0.1) * np.float32(cfg.time_step)))
+    d0 = o.a["pos_h"][0].astype(np.float64) - p0
+    d1 = o.a["pos_h"][1].astype(np.float64) - p1
+    np.testing.assert_allclose(d0, dt_r * dt_r * a0, rtol=2e-3, atol=2e-9)
+    np.testing.assert_allclose(d1, -dt_r * dt_r * a0, rtol=2e-3, atol=2e-9)
+    np.testing.assert_allclose(o.a["vel_h"][0], dt_r * a0, rtol=1e-5)
+    # negative pressure (two particles are far below rest density) attracts
+    assert p < 0 and float(d0 @ X) < 0

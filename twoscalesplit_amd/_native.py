@@ -58,6 +58,8 @@ class TssConfig(C.Structure):
         ("dt_mode", C.c_int),
         ("cfl", C.c_float),
         ("dt_max", C.c_float),
+        ("relax_iters", C.c_int),
+        ("relax_dt_frac", C.c_float),
     ]
 
 

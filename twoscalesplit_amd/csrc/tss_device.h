@@ -70,7 +70,8 @@ struct SceneDev {
     float cfl, dt_max;
     float dt;                 // dt of the next step (device-updated)
     int alive;
-    int pad[2];
+    int relax_iters;          // relaxation iterations of new children
+    float relax_frac;         // relaxation step = relax_frac * dt
 };
 
 struct SceneCount {
@@ -91,6 +92,9 @@ struct DevCounts {
     int n_l, n_h, n_g;        // lists the next step reads
     int n_l_next, n_h_next, n_g_next;
     int n_l_last, n_h_last;   // lists the last step advanced (sorted sizes)
+    // children injected by the last step's splits: the last n_fresh entries
+    // of the high list (new children are appended after the kept ones)
+    int n_fresh, n_fresh_next;
 };
 
 // Device buffers of a batch (all pointers are device memory).
@@ -142,6 +146,8 @@ struct DevState {
     int* red_n;               // [S] chunk slots of each scene
     double* cons;             // [S * 10] conservation sums + bounds
     uint8_t* cflag;           // [2 (CCL + CCH)] cflag[2c + s] = 1: state s in cell c
+    float4* rtmp;             // [4 CL] relaxation scratch (x', v') per target;
+                              // allocated when a scene enables relaxation
 };
 
 // Launch shape of one neighbour kernel: block size (128, 256, 512), LDS
@@ -164,6 +170,7 @@ struct StepShape {
     int S;
     int red_slots;            // chunk slots in use (grid of the reduction)
     int list_parity;          // which list buffer holds the current lists
+    int relax_iters;          // max relaxation iterations over the scenes
 };
 
 // Timing points of one step (events recorded before / after each phase):

@@ -241,6 +241,7 @@ struct tss_batch {
         b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) + 10 * 8 + 12);
         b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
         b += 2 * (ds.cells_low + ds.cells_high);  // cell state flags
+        b += 16 * 4 * CL;                         // relaxation scratch (when used)
         return b;
     }
 
@@ -395,6 +396,11 @@ struct tss_batch {
         sh.key_bits = bits;
         sh.passes = (bits + 7) / 8;
         for (int k = 0; k < tss::kPairKernels; ++k) sh.tune[k] = tune[k];
+        sh.relax_iters = 0;
+        for (auto& s : scenes)
+            if (s.alive) sh.relax_iters = std::max(sh.relax_iters, s.cfg.relax_iters);
+        if (sh.relax_iters > 0 && !d.rtmp)  // between steps: never under capture
+            d.rtmp = dalloc<float4>((size_t)4 * cap_l, owned);
         invalidate_graphs();
     }
 
@@ -489,6 +495,8 @@ struct tss_batch {
         s.dt_max = c.dt_max;
         s.dt = c.time_step;
         s.alive = 1;
+        s.relax_iters = c.relax_iters;
+        s.relax_frac = c.relax_dt_frac;
         return s;
     }
 
@@ -513,6 +521,9 @@ struct tss_batch {
         require(c.dt_mode == TSS_DT_FIXED || (c.dt_mode == TSS_DT_CFL && c.cfl > 0 && c.dt_max > 0),
                 "invalid time-step control");
         require(c.time_step > 0, "time step must be > 0");
+        require(c.relax_iters >= 0 && c.relax_iters <= 64, "relax_iters must be within 0..64");
+        require(c.relax_iters == 0 || (c.relax_dt_frac > 0 && c.relax_dt_frac <= 1),
+                "relax_dt_frac must be within (0, 1]");
     }
 
     int add_scene(const tss_config& c, const float* low_xy, int n_low, int cap_low,
@@ -1117,6 +1128,7 @@ struct tss_batch {
         c.n_l = nl;
         c.n_h = nh;
         c.n_g = ng;
+        c.n_fresh = 0;  // a re-seeded state has no children awaiting relaxation
         write_counts(c);
         tss::SceneCount sc = read_scnt(sid);
         sc.n_l = nl;
@@ -1308,6 +1320,8 @@ int tss_config_init(tss_config* cfg, float xs, float ys, float xe, float ye, flo
         cfg->dt_mode = TSS_DT_FIXED;
         cfg->cfl = 0.4f;
         cfg->dt_max = dt;
+        cfg->relax_iters = 0;     // relaxation off
+        cfg->relax_dt_frac = 0.1f;
     });
 }
 

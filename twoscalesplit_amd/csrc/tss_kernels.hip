@@ -430,6 +430,156 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
 #include "tss_pairs.inc"
 
 // ---------------------------------------------------------------------------
+// Relaxation of the children injected by the last step's splits (the last
+// n_fresh entries of the high list): relax_iters Jacobi iterations, each
+//   1. density + Tait of every target, with the high level's own sums
+//      (lambda_HH over high r <= H, lambda_HL over low r <= H, blended M4);
+//   2. pressure + wall acceleration of every target (no viscosity, tension
+//      or gravity): -m_H sum_HH (p_i/rho_i^2 + p_j/rho_j^2) grad W_bar over
+//      0 < r < H, -m_L sum_HL (...) over 0 < r < h, + walls; then
+//      v' = v + dt_r a, x' = x + dt_r v' with dt_r = relax_frac * dt, into
+//      scratch (every target reads the iteration's start state);
+//   3. commit of x', v' to the by-id and cell-sorted arrays.
+// Neighbours' rho and p are read by id: the targets' from step 1, everyone
+// else's from the last step. Cells stay those of the grid just built.
+// (TwoScale2D-master/TwoScale2D/WCSPHSolver.cu:2380-2446, 1133-1230,
+// 1499-1625, dormant at R:3044-3110; here with the method's weights and
+// without the in-place race.)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool relax_target(const DevState& d,
+                                             const int* __restrict__ lh, int t,
+                                             int it, int& u, int& k) {
+    const int nh = d.counts->n_h;
+    const int nf = min(d.counts->n_fresh, nh);
+    u = lh[nh - nf + t];
+    k = d.sidx[u];
+    return it < d.scenes[d.sscene[k]].relax_iters;
+}
+
+__global__ void k_relax_density(DevState d, const int* __restrict__ lh,
+                                int it) {
+    GRID_STRIDE(t, min(d.counts->n_fresh, d.counts->n_h)) {
+        int u, k;
+        if (!relax_target(d, lh, t, it, u, k)) continue;
+        const SceneDev& sc = d.scenes[d.sscene[k]];
+        const float2 xi = d.spos[k];
+        const uint8_t si = d.sstate[k];
+        const bool i_prom = !(si & 1u);
+        float s1 = 0.f, c1 = 0.f, s2 = 0.f, c2 = 0.f;
+        Box b = scan_box(sc, true, xi, sc.H);
+        for (int cy = b.y0; cy <= b.y1; ++cy) {
+            int st, en;
+            row_run(d.cstart, d.cend, sc.cbase_h + cy * sc.dhx, b.x0, b.x1, st, en);
+            for (int j = st; j < en; ++j) {
+                const float r2 = dist2(xi, d.spos[j]);
+                if (r2 <= sc.t_le_H && !(i_prom && (d.sstate[j] & 1u))) {
+                    float w, g;
+                    m4_blend(sc, dist_from2(r2), w, g);
+                    kahan_add(s1, c1, w);
+                }
+            }
+        }
+        b = scan_box(sc, false, xi, sc.h);
+        for (int cy = b.y0; cy <= b.y1; ++cy) {
+            int st, en;
+            row_run(d.cstart, d.cend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1, st, en);
+            for (int j = st; j < en; ++j) {
+                const float r2 = dist2(xi, d.spos[j]);
+                if (r2 <= sc.t_le_H && !((si & 1u) && (d.sstate[j] & 1u))) {
+                    float w, g;
+                    m4_blend(sc, dist_from2(r2), w, g);
+                    kahan_add(s2, c2, w);
+                }
+            }
+        }
+        float hi, lo;
+        mass_sum(sc.m_h, s1, c1, sc.m_l, s2, c2, hi, lo);
+        d.rho[u] = hi + lo;
+        d.p[u] = tait(sc, hi, lo);
+    }
+}
+
+__global__ void k_relax_force(DevState d, const uint32_t* __restrict__ vals,
+                              const int* __restrict__ lh, int it) {
+    GRID_STRIDE(t, min(d.counts->n_fresh, d.counts->n_h)) {
+        int u, k;
+        if (!relax_target(d, lh, t, it, u, k)) continue;
+        const SceneDev& sc = d.scenes[d.sscene[k]];
+        const float4 xv = d.sxv[k];
+        const float2 xi = make_float2(xv.x, xv.y);
+        const uint8_t si = d.sstate[k];
+        const bool i_prom = !(si & 1u);
+        const float rhoi = d.rho[u];
+        const float pri = d.p[u] / (rhoi * rhoi);
+        float sx = 0.f, cx = 0.f, sy = 0.f, cy_ = 0.f;
+        auto pair = [&](int j, float mneg) {
+            const float2 xj = d.spos[j];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y;
+            const float r2 = fmaf(dx, dx, dy * dy);
+            const float rinv = fast_rsq(r2);
+            float w, g;
+            m4_blend(sc, r2 * rinv, w, g);
+            const int uj = (int)vals[j];
+            const float rhoj = d.rho[uj];
+            const float coeff = pri + d.p[uj] / (rhoj * rhoj);
+            const float f = mneg * coeff * g * rinv;
+            kahan_add(sx, cx, f * dx);
+            kahan_add(sy, cy_, f * dy);
+        };
+        Box b = scan_box(sc, true, xi, sc.H);
+        for (int ry = b.y0; ry <= b.y1; ++ry) {
+            int st, en;
+            row_run(d.cstart, d.cend, sc.cbase_h + ry * sc.dhx, b.x0, b.x1, st, en);
+            for (int j = st; j < en; ++j) {
+                const float r2 = dist2(xi, d.spos[j]);
+                if (in_open_ball(r2, sc.t_lt_H) && !(i_prom && (d.sstate[j] & 1u)))
+                    pair(j, -sc.m_h);
+            }
+        }
+        b = scan_box(sc, false, xi, sc.h);
+        float bx = 0.f, by = 0.f;
+        for (int ry = b.y0; ry <= b.y1; ++ry) {
+            int st, en;
+            row_run(d.cstart, d.cend, sc.cbase_l + ry * sc.dlx, b.x0, b.x1, st, en);
+            for (int j = st; j < en; ++j) {
+                const float r2 = dist2(xi, d.spos[j]);
+                const uint8_t sj = d.sstate[j];
+                if (in_open_ball(r2, sc.t_lt_h) && !((si & 1u) && (sj & 1u)) && sj != kDeleted)
+                    pair(j, -sc.m_l);
+            }
+            int bs, be;
+            row_run(d.bstart, d.bend, sc.cbase_l + ry * sc.dlx, b.x0, b.x1, bs, be);
+            for (int j = bs; j < be; ++j) {
+                const float2 xb = d.bpos[j];
+                const float r2 = dist2(xi, xb);
+                const float gm = sc.cb_h * wall_gamma(sc, r2, dist_from2(r2), sc.q_H);
+                bx = fmaf(gm, xi.x - xb.x, bx);
+                by = fmaf(gm, xi.y - xb.y, by);
+            }
+        }
+        const float ax = (sx - cx) + bx;
+        const float ay = (sy - cy_) + by;
+        const float dt = sc.relax_frac * sc.dt;
+        const float vx = xv.z + dt * ax;
+        const float vy = xv.w + dt * ay;
+        d.rtmp[t] = make_float4(xi.x + dt * vx, xi.y + dt * vy, vx, vy);
+    }
+}
+
+__global__ void k_relax_commit(DevState d, const int* __restrict__ lh, int it) {
+    GRID_STRIDE(t, min(d.counts->n_fresh, d.counts->n_h)) {
+        int u, k;
+        if (!relax_target(d, lh, t, it, u, k)) continue;
+        const float4 v = d.rtmp[t];
+        d.pos[u] = make_float2(v.x, v.y);
+        d.vel[u] = make_float2(v.z, v.w);
+        d.spos[k] = make_float2(v.x, v.y);
+        d.sxv[k] = v;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
 // FP64 reductions per scene: conservation sums over real particles (listed
 // low, promoted high) and the max |v|^2 for the CFL time step. Fixed chunk
 // and tree shape -> bitwise reproducible, independent of other scenes.
@@ -818,6 +968,7 @@ __global__ __launch_bounds__(1024) void k_compact_top(DevState d, StepShape sh) 
         d.seg_base[7] = nmerge;
         d.counts->n_l_next = keepL + nmerge;
         d.counts->n_h_next = keepH + 4 * nsplit;
+        d.counts->n_fresh_next = 4 * nsplit;
         d.counts->n_g_next = keepG + nsplit;
     }
 }
@@ -975,6 +1126,7 @@ __global__ __launch_bounds__(256) void k_finalize(DevState d) {
         g.n_l = g.n_l_next;
         g.n_h = g.n_h_next;
         g.n_g = g.n_g_next;
+        g.n_fresh = g.n_fresh_next;
     }
 }
 
@@ -1134,12 +1286,17 @@ __global__ __launch_bounds__(1024) void k_filter_lists(DevState d, int scene,
                                                       int* __restrict__ lg) {
     __shared__ int part[1024];
     __shared__ int base;
+    __shared__ int fresh;  // kept entries of the high list's fresh tail
     int* lists[3] = {ll, lh, lg};
     int* ns[3] = {&d.counts->n_l, &d.counts->n_h, &d.counts->n_g};
     for (int which = 0; which < 3; ++which) {
         int* L = lists[which];
         const int n = *ns[which];
-        if (threadIdx.x == 0) base = 0;
+        const int tail0 = which == 1 ? n - d.counts->n_fresh : n;
+        if (threadIdx.x == 0) {
+            base = 0;
+            fresh = 0;
+        }
         __syncthreads();
         for (int b0 = 0; b0 < n; b0 += 1024) {
             int i = b0 + threadIdx.x;
@@ -1159,6 +1316,7 @@ __global__ __launch_bounds__(1024) void k_filter_lists(DevState d, int scene,
             }
             int pos = base + part[threadIdx.x] - keep;
             int tot = part[1023];
+            if (keep && i >= tail0) atomicAdd(&fresh, 1);
             __syncthreads();
             // in-place is safe: pos <= i and every read of this chunk
             // happened before the barriers above
@@ -1167,7 +1325,10 @@ __global__ __launch_bounds__(1024) void k_filter_lists(DevState d, int scene,
             if (threadIdx.x == 0) base += tot;
             __syncthreads();
         }
-        if (threadIdx.x == 0) *ns[which] = base;
+        if (threadIdx.x == 0) {
+            *ns[which] = base;
+            if (which == 1) d.counts->n_fresh = fresh;
+        }
         __syncthreads();
     }
 }
@@ -1242,6 +1403,12 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     mark(0);
     const int cur = enqueue_grid(d, sh, st);
     const uint32_t* vals = d.vals[cur];
+    for (int it = 0; it < sh.relax_iters; ++it) {
+        const int gr = blocks_for(4LL * d.CL, 256, 4096);
+        k_relax_density<<<gr, 256, 0, st>>>(d, d.list_h[par], it);
+        k_relax_force<<<gr, 256, 0, st>>>(d, vals, d.list_h[par], it);
+        k_relax_commit<<<gr, 256, 0, st>>>(d, d.list_h[par], it);
+    }
     mark(1);
     launch_pair<false, false>(d, vals, sh.tune[kTuneDensityLow], d.CL, st);
     mark(2);
