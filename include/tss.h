@@ -289,6 +289,22 @@ int tss_batch_autotune(tss_batch* b, int steps_per_trial, int rounds,
                        int log_rows, int* n_log);
 void tss_batch_destroy(tss_batch* b);
 
+/* Scene generated on the device: the seeded dam break of
+ * tss_scene_dam_break below (fluid positions bit for bit identical), with
+ * the fluid block written by a kernel instead of copied from the host; the
+ * wall canvas (perimeter-sized) is built on the host. Replaces
+ * CreateParticleBox / CreateParticleBoxCanvas (R/ParticleSystem.cpp:123-179),
+ * which upload one particle per copy (R/ParticleSystem.cpp:73-74). */
+typedef struct tss_dam_break {
+    uint64_t seed, scene;
+    int nx, ny;
+    float dx, jitter;
+    int nxd, nyd, layers;
+} tss_dam_break;
+int tss_batch_add_dam_break(tss_batch* b, const tss_config* cfg,
+                            const tss_dam_break* gen, int cap_low,
+                            int* scene_id);
+
 /* ---- serving loop ---------------------------------------------------------- */
 /* A scene request: its initial state, the steps it asks for and its arrival
  * time; optional output buffers receive fields of its final state. No
@@ -300,6 +316,9 @@ typedef struct tss_request {
     int n_low, cap_low;
     const float* bnd_xy;
     int n_bnd;
+    const tss_dam_break* gen;       /* or NULL; when set, the scene is
+                                       generated on the device at admission
+                                       and low_xy / bnd_xy are not read     */
     int steps;                      /* steps to advance (>= 1)            */
     double arrival_s;               /* seconds after the start of the run */
     void* out[TSS_FIELD_COUNT];     /* optional: field copies at finish   */

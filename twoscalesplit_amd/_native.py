@@ -96,12 +96,23 @@ class TssTuning(C.Structure):
         return t
 
 
+class TssDamBreak(C.Structure):
+    """Seeded dam-break scene that the device generates (tss_dam_break)."""
+    _fields_ = [
+        ("seed", C.c_uint64), ("scene", C.c_uint64),
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("dx", C.c_float), ("jitter", C.c_float),
+        ("nxd", C.c_int), ("nyd", C.c_int), ("layers", C.c_int),
+    ]
+
+
 class TssRequest(C.Structure):
     """One scene request of tss_serve."""
     _fields_ = [
         ("cfg", C.POINTER(TssConfig)),
         ("low_xy", C.c_void_p), ("n_low", C.c_int), ("cap_low", C.c_int),
         ("bnd_xy", C.c_void_p), ("n_bnd", C.c_int),
+        ("gen", C.POINTER(TssDamBreak)),
         ("steps", C.c_int),
         ("arrival_s", C.c_double),
         ("out", C.c_void_p * len(FIELDS)),
@@ -170,6 +181,9 @@ PROTOTYPES = {
     "tss_batch_create": (_I, [C.POINTER(TssBatchDesc), C.POINTER(_P)]),
     "tss_batch_add_scene": (_I, [_P, C.POINTER(TssConfig), _P, _I, _I, _P, _I,
                                  C.POINTER(_I)]),
+    "tss_batch_add_dam_break": (_I, [_P, C.POINTER(TssConfig),
+                                     C.POINTER(TssDamBreak), _I,
+                                     C.POINTER(_I)]),
     "tss_batch_remove_scene": (_I, [_P, _I]),
     "tss_batch_step": (_I, [_P, _I]),
     "tss_batch_step_graph": (_I, [_P, _I]),

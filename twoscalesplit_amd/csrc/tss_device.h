@@ -195,6 +195,10 @@ void launch_grid_only(const DevState& d, const StepShape& sh, void* stream);
 void launch_work_counts(const DevState& d, const StepShape& sh,
                         unsigned long long* out, void* stream);
 void launch_fill_int(int* p, int value, long long n, void* stream);
+void launch_iota(int* p, int first, int n, void* stream);
+void launch_dam_break(float2* pos, int nx, int ny, float dx, float jitter,
+                      unsigned long long seed, unsigned long long scene,
+                      void* stream);
 void launch_copy16(const void* src, void* dst, long long n16, void* stream);
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
                          void* stream);

@@ -526,11 +526,21 @@ struct tss_batch {
                 "relax_dt_frac must be within (0, 1]");
     }
 
+    // low_xy == nullptr with gen: the fluid block is generated on the device
+    // (tss_dam_break), bit for bit the positions of tss_scene_dam_break
     int add_scene(const tss_config& c, const float* low_xy, int n_low, int cap_low,
-                  const float* bnd_xy, int n_bnd) {
+                  const float* bnd_xy, int n_bnd, const tss_dam_break* gen = nullptr) {
         validate(c, n_low, cap_low, n_bnd);
-        for (int i = 0; i < 2 * n_low; ++i)
-            require(std::isfinite(low_xy[i]), "non-finite particle position");
+        require((low_xy != nullptr) != (gen != nullptr) || n_low == 0,
+                "a scene takes positions or a generator");
+        if (gen) {
+            require(gen->nx >= 0 && gen->ny >= 0 && (long long)gen->nx * gen->ny == n_low &&
+                        gen->dx > 0 && gen->jitter >= 0 && std::isfinite(gen->dx),
+                    "invalid scene generator");
+        } else {
+            for (int i = 0; i < 2 * n_low; ++i)
+                require(std::isfinite(low_xy[i]), "non-finite particle position");
+        }
         for (int i = 0; i < 2 * n_bnd; ++i)
             require(std::isfinite(bnd_xy[i]), "non-finite boundary position");
         int sid = -1;
@@ -581,8 +591,12 @@ struct tss_batch {
         const int CL = d.CL;
         const int lbase = sh_.lbase;
         // particle state
-        HIP_CHECK(hipMemcpyAsync(d.pos + lbase, low_xy, sizeof(float) * 2 * (size_t)n_low,
-                                 hipMemcpyHostToDevice, stream));
+        if (gen)
+            tss::launch_dam_break(d.pos + lbase, gen->nx, gen->ny, gen->dx, gen->jitter,
+                                  gen->seed, gen->scene, stream);
+        else
+            HIP_CHECK(hipMemcpyAsync(d.pos + lbase, low_xy, sizeof(float) * 2 * (size_t)n_low,
+                                     hipMemcpyHostToDevice, stream));
         if (cap_low > n_low)
             HIP_CHECK(hipMemsetAsync(d.pos + lbase + n_low, 0, sizeof(float2) * (size_t)(cap_low - n_low), stream));
         HIP_CHECK(hipMemsetAsync(d.vel + lbase, 0, sizeof(float2) * (size_t)cap_low, stream));
@@ -630,12 +644,8 @@ struct tss_batch {
         HIP_CHECK(hipMemcpyAsync(d.bend + cl, ben.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, stream));
         // initial low list: ids in lattice order (the reference's iota list,
         // R/WCSPHSolver.cu:2358-2371), appended to the current list
-        std::vector<int> ids(n_low);
-        for (int i = 0; i < n_low; ++i) ids[i] = lbase + i;
         const int par = sh.list_parity;
-        if (n_low)
-            HIP_CHECK(hipMemcpyAsync(d.list_l[par] + cnt.n_l, ids.data(), sizeof(int) * n_low,
-                                     hipMemcpyHostToDevice, stream));
+        tss::launch_iota(d.list_l[par] + cnt.n_l, lbase, n_low, stream);
         cnt.n_l += n_low;
         tss::SceneCount sc{};
         sc.n_l = n_low;
@@ -1469,6 +1479,30 @@ int tss_batch_add_scene(tss_batch* b, const tss_config* cfg, const float* low_xy
     });
 }
 
+// wall canvas of a generated scene (host; perimeter-sized)
+static std::vector<float> generated_walls(const tss_dam_break& g) {
+    const int nb = tss_canvas_count(g.nxd, g.nyd, g.layers);
+    require(nb >= 0, "invalid scene generator canvas");
+    std::vector<float> walls(2 * (size_t)nb + 2);
+    const int rc = tss_scene_dam_break(g.seed, g.scene, 0, 0, g.dx, 0.0f, g.nxd, g.nyd,
+                                       g.layers, nullptr, walls.data());
+    if (rc != TSS_OK) throw TssError(rc, tss_last_error());
+    walls.resize(2 * (size_t)nb);
+    return walls;
+}
+
+int tss_batch_add_dam_break(tss_batch* b, const tss_config* cfg, const tss_dam_break* g,
+                            int cap_low, int* scene_id) {
+    return guarded([&] {
+        require(b && cfg && g && scene_id, "null argument");
+        require(g->nx >= 0 && g->ny >= 0 && (long long)g->nx * g->ny <= cap_low,
+                "scene generator larger than cap_low");
+        std::vector<float> walls = generated_walls(*g);
+        *scene_id = b->add_scene(*cfg, nullptr, g->nx * g->ny, cap_low, walls.data(),
+                                 (int)(walls.size() / 2), g);
+    });
+}
+
 int tss_batch_remove_scene(tss_batch* b, int sid) {
     return guarded([&] {
         require(b, "null argument");
@@ -1652,7 +1686,7 @@ int tss_serve(const tss_batch_desc* cap, long long budget, tss_request* req, int
             q.status = TSS_E_STATE;  // pending
             q.admit_s = q.finish_s = -1.0;
             q.particle_steps = 0;
-            if (q.steps < 1 || !q.cfg || !q.low_xy) {
+            if (q.steps < 1 || !q.cfg || (!q.low_xy && !q.gen && q.n_low > 0)) {
                 q.status = TSS_E_INVALID;
                 ++done;
             }
@@ -1678,7 +1712,9 @@ int tss_serve(const tss_batch_desc* cap, long long budget, tss_request* req, int
             bool changed = false;
             for (auto it = queue.begin(); it != queue.end();) {
                 tss_request& q = req[*it];
-                const long long sb = scene_bytes(*q.cfg, q.cap_low, q.n_bnd);
+                const int nbq = q.gen ? tss_canvas_count(q.gen->nxd, q.gen->nyd, q.gen->layers)
+                                      : q.n_bnd;
+                const long long sb = scene_bytes(*q.cfg, q.cap_low, nbq < 0 ? 0 : nbq);
                 if (sb > budget || sb > st->batch_bytes) {  // can never fit
                     q.status = TSS_E_NOMEM;
                     q.finish_s = now();
@@ -1692,7 +1728,14 @@ int tss_serve(const tss_batch_desc* cap, long long budget, tss_request* req, int
                 }
                 int sid = -1;
                 try {
-                    sid = b->add_scene(*q.cfg, q.low_xy, q.n_low, q.cap_low, q.bnd_xy, q.n_bnd);
+                    if (q.gen) {
+                        std::vector<float> walls = generated_walls(*q.gen);
+                        sid = b->add_scene(*q.cfg, nullptr, q.gen->nx * q.gen->ny, q.cap_low,
+                                           walls.data(), (int)(walls.size() / 2), q.gen);
+                    } else {
+                        sid = b->add_scene(*q.cfg, q.low_xy, q.n_low, q.cap_low, q.bnd_xy,
+                                           q.n_bnd);
+                    }
                 } catch (const TssError& e) {
                     if (e.code == TSS_E_NOMEM && !res.empty()) {  // wait for room
                         ++it;

@@ -1357,6 +1357,53 @@ void launch_copy16(const void* src, void* dst, long long n16, void* stream) {
         (const float4*)src, (float4*)dst, n16);
 }
 
+// Seeded dam-break block on the device: particle idx = j * nx + i at
+// ((0.5 + i) dx, (0.5 + j) dx) + (2u - 1) * amp per axis, u = the top 24 bits
+// of SplitMix64 outputs keyed by (seed, scene, idx) -- the same arithmetic,
+// operation by operation, as the host generator tss_scene_dam_break.
+__device__ __forceinline__ uint64_t mix64_dev(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_dam_break(float2* __restrict__ pos, int nx, int n,
+                            float dx, float amp, uint64_t seed,
+                            uint64_t scene) {
+    constexpr uint64_t kG = 0x9E3779B97F4A7C15ull;
+    GRID_STRIDE(idx, n) {
+        const int j = idx / nx, i = idx - j * nx;
+        uint64_t st = mix64_dev(mix64_dev(seed + kG * (scene + 1)) +
+                                kG * ((uint64_t)idx + 1));
+        st += kG;
+        const float ux = (float)(mix64_dev(st) >> 40) * (1.0f / 16777216.0f);
+        st += kG;
+        const float uy = (float)(mix64_dev(st) >> 40) * (1.0f / 16777216.0f);
+        const float jx = (2.0f * ux - 1.0f) * amp;
+        const float jy = (2.0f * uy - 1.0f) * amp;
+        pos[idx] = make_float2((0.5f + (float)i) * dx + jx,
+                               (0.5f + (float)j) * dx + jy);
+    }
+}
+
+__global__ void k_iota(int* __restrict__ p, int first, int n) {
+    GRID_STRIDE(i, n) p[i] = first + i;
+}
+
+void launch_dam_break(float2* pos, int nx, int ny, float dx, float jitter,
+                      unsigned long long seed, unsigned long long scene,
+                      void* stream) {
+    const int n = nx * ny;
+    if (n <= 0) return;
+    k_dam_break<<<blocks_for(n, 256, 4096), 256, 0, (hipStream_t)stream>>>(
+        pos, nx, n, dx, jitter * dx, seed, scene);
+}
+
+void launch_iota(int* p, int first, int n, void* stream) {
+    if (n <= 0) return;
+    k_iota<<<blocks_for(n, 256, 4096), 256, 0, (hipStream_t)stream>>>(p, first, n);
+}
+
 void launch_fill_int(int* p, int value, long long n, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     k_fill_int<<<blocks_for(n, 256, 4096), 256, 0, st>>>(p, value, n);

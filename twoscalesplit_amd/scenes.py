@@ -43,6 +43,7 @@ class Scene:
     cap_low: int
     steps: int
     meta: dict = field(default_factory=dict)
+    gen: N.TssDamBreak | None = None    # the same scene as a device generator
 
     @property
     def n_low(self) -> int:
@@ -53,6 +54,13 @@ class Scene:
         c = self.cfg
         return (c.domain_dims[0] * c.domain_dims[1],
                 c.domain_dims_high[0] * c.domain_dims_high[1])
+
+
+def spec(seed: int, scene: int, nx: int, ny: int, dx: float, nxd: int,
+         nyd: int, layers: int = 3, jitter: float = 0.01) -> N.TssDamBreak:
+    """The arguments of ``generate`` as a device generator spec."""
+    return N.TssDamBreak(seed, scene, nx, ny, float(dx), float(jitter), nxd,
+                         nyd, layers)
 
 
 def generate(seed: int, scene: int, nx: int, ny: int, dx: float, nxd: int,
@@ -99,7 +107,7 @@ def c1_plain(seed: int = SEED) -> Scene:
     cfg = _config(1.0, 1.0, dx, two_scale=False)
     cfg.mode = N.MODE_PLAIN
     return Scene("C1 plain 2D dam break (8192 low)", cfg, low, bnd, len(low),
-                 1000, {"dx": dx})
+                 1000, {"dx": dx}, spec(seed, 0, 64, 128, dx, 160, 160))
 
 
 def c2_two_scale(seed: int = SEED, cap_high: int = 32768) -> Scene:
@@ -111,7 +119,8 @@ def c2_two_scale(seed: int = SEED, cap_high: int = 32768) -> Scene:
     cfg.merge_radius = float(np.float32(1.5) * np.float32(cfg.effective_radius))
     cfg.cap_high = cap_high
     return Scene("C2 two-scale dam break (8192 low + <=32768 high)", cfg, low,
-                 bnd, len(low), 2000, {"dx": dx})
+                 bnd, len(low), 2000, {"dx": dx},
+                 spec(seed, 0, 64, 128, dx, 160, 160))
 
 
 def c3_large(seed: int = SEED, forced: bool = True) -> Scene:
@@ -135,7 +144,8 @@ def c3_large(seed: int = SEED, forced: bool = True) -> Scene:
             f"({nx * ny} low + {4 * nx * ny} high)" if forced else
             f"C3 large two-scale dam break ({nx * ny} low, band)")
     return Scene(name, cfg, low, bnd, nx * ny, 500,
-                 {"dx": dx, "forced": forced})
+                 {"dx": dx, "forced": forced},
+                 spec(seed, 0, nx, ny, dx, nxd, nyd))
 
 
 def _scene_from_count(seed: int, idx: int, n_low: int, aspect: float,
@@ -153,7 +163,8 @@ def _scene_from_count(seed: int, idx: int, n_low: int, aspect: float,
     cfg.merge_radius = float(np.float32(1.5) * np.float32(cfg.effective_radius))
     cfg.cap_high = 4 * nx * ny
     return Scene(f"scene {idx} ({nx}x{ny})", cfg, low, bnd, nx * ny, steps,
-                 {"dx": dx, "aspect": aspect, "alpha": alpha})
+                 {"dx": dx, "aspect": aspect, "alpha": alpha},
+                 spec(seed, idx, nx, ny, dx, nxd, nyd))
 
 
 def c4_batched(seed: int = SEED, n_scenes: int = 256, lo: int = 2000,
@@ -203,4 +214,4 @@ def small_two_scale(seed: int = SEED, nx: int = 16, ny: int = 24,
     cfg.cap_high = 4 * nx * ny if mode != N.MODE_PLAIN else 0
     cfg.merge_radius = float(np.float32(1.5) * np.float32(cfg.effective_radius))
     return Scene(f"small {nx}x{ny} mode {mode}", cfg, low, bnd, nx * ny, 0,
-                 {"dx": dx})
+                 {"dx": dx}, spec(seed, 7, nx, ny, dx, nxd, nyd))

@@ -68,9 +68,12 @@ def pool_desc(trace: Trace, max_resident_low: int, max_scenes: int = 256
 
 
 def serve(trace: Trace, desc: N.TssBatchDesc, budget: int = BUDGET_BYTES,
-          outputs: tuple[str, ...] = ()) -> tuple[dict, list[dict], list]:
+          outputs: tuple[str, ...] = (), device_gen: bool = False
+          ) -> tuple[dict, list[dict], list]:
     """Run the trace through tss_serve. Returns (stats, per-request results,
-    per-request output arrays)."""
+    per-request output arrays). ``device_gen``: requests carry their scene's
+    generator spec instead of position arrays, and the device generates the
+    fluid block at admission."""
     n = len(trace.scenes)
     reqs = (N.TssRequest * n)()
     keep = []
@@ -86,6 +89,13 @@ def serve(trace: Trace, desc: N.TssBatchDesc, budget: int = BUDGET_BYTES,
         r.cap_low = s.cap_low
         r.bnd_xy = bnd.ctypes.data
         r.n_bnd = len(bnd)
+        if device_gen:
+            if s.gen is None:
+                raise ValueError(f"{s.name} has no generator spec")
+            r.gen = C.pointer(s.gen)
+            r.low_xy = None
+            r.bnd_xy = None
+            r.n_bnd = 0
         r.steps = s.steps
         r.arrival_s = float(trace.arrival[i])
         o = {}

@@ -256,6 +256,17 @@ class Batch:
                 "tss_batch_add_scene")
         return sid.value
 
+    def add_dam_break(self, cfg: N.TssConfig, gen: N.TssDamBreak,
+                      cap_low: int) -> int:
+        """A seeded dam-break scene whose fluid block the device generates
+        (positions bit for bit those of ``scenes.generate``)."""
+        sid = C.c_int()
+        N.check(N.lib().tss_batch_add_dam_break(self._h, C.byref(cfg),
+                                                C.byref(gen), int(cap_low),
+                                                C.byref(sid)),
+                "tss_batch_add_dam_break")
+        return sid.value
+
     def remove_scene(self, sid: int) -> None:
         N.check(N.lib().tss_batch_remove_scene(self._h, sid),
                 "tss_batch_remove_scene")
