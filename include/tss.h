@@ -305,6 +305,57 @@ int tss_batch_add_dam_break(tss_batch* b, const tss_config* cfg,
                             const tss_dam_break* gen, int cap_low,
                             int* scene_id);
 
+/* ---- headless frame export -------------------------------------------------- */
+/* One frame of a scene: the particles of its current lists -- exactly the
+ * points the reference draws (glDrawElements over the ParticleIDs index
+ * buffer, R/QuantityRenderer.cpp:84-95, with the positions and visual
+ * quantities of R/ParticleSystem.cpp:21-46 mapped into CUDA by Map/Unmap,
+ * R/ParticleSystem.cpp:78-98) -- gathered on the device in list order and
+ * copied out in one transfer, replacing the GL interop with a buffer that a
+ * file, a pipe or an offline renderer can take.
+ *
+ * Layout: a 64-byte tss_frame_header, then the low section, then the high
+ * section, then (TSS_FRAME_BND) the boundary positions by id. A level's
+ * section holds one array per requested field, in bit order
+ * (ID, POS, VEL, RHO, P, VIS, STATE), each n entries long and padded with
+ * zero bytes to a multiple of 16 bytes. IDs are scene-local (low id, or high
+ * id = 4 * parent + k); STATE is one byte per particle. Frames are
+ * self-describing, so a stream of frames is their concatenation. */
+enum tss_frame_bits {
+    TSS_FRAME_ID = 1,          /* int32  */
+    TSS_FRAME_POS = 2,         /* float2 */
+    TSS_FRAME_VEL = 4,         /* float2 */
+    TSS_FRAME_RHO = 8,         /* float  */
+    TSS_FRAME_P = 16,          /* float  */
+    TSS_FRAME_VIS = 32,        /* float: min(|rho - rho0|, 400) / 400 */
+    TSS_FRAME_STATE = 64,      /* uint8  */
+    TSS_FRAME_BND = 128,       /* boundary positions (float2 by id) */
+    TSS_FRAME_ALL = 255
+};
+#define TSS_FRAME_MAGIC 0x46535354u  /* bytes "TSSF" */
+#define TSS_FRAME_VERSION 1u
+typedef struct tss_frame_header {
+    uint32_t magic;            /* TSS_FRAME_MAGIC */
+    uint32_t version;          /* TSS_FRAME_VERSION */
+    uint32_t fields;           /* tss_frame_bits of the sections present */
+    int32_t step;              /* completed steps of the scene */
+    int32_t n_low, n_high;     /* entries of the low and high sections */
+    int32_t n_bnd;             /* boundary entries (0 without TSS_FRAME_BND) */
+    float dt;                  /* dt of the next step */
+    float domain_origin[2];
+    float domain_end[2];
+    uint32_t reserved[4];      /* zero */
+} tss_frame_header;
+/* Bytes of a frame with these fields and counts (0 for invalid arguments). */
+size_t tss_frame_size(uint32_t fields, int n_low, int n_high, int n_bnd);
+/* Writes the current frame of the scene into dst (synchronising; between
+ * steps). *bytes receives the frame's size; with dst == NULL nothing else
+ * happens (a size query). TSS_E_INVALID if cap is smaller than the frame. */
+int tss_frame(tss_solver* s, uint32_t fields, void* dst, size_t cap,
+              size_t* bytes);
+int tss_batch_frame(tss_batch* b, int scene_id, uint32_t fields, void* dst,
+                    size_t cap, size_t* bytes);
+
 /* ---- serving loop ---------------------------------------------------------- */
 /* A scene request: its initial state, the steps it asks for and its arrival
  * time; optional output buffers receive fields of its final state. No

@@ -189,6 +189,9 @@ PROTOTYPES = {
     "tss_batch_step_graph": (_I, [_P, _I]),
     "tss_batch_counts": (_I, [_P, _I, C.POINTER(TssCounts)]),
     "tss_batch_read": (_I, [_P, _I, _I, _P, _SZ]),
+    "tss_frame_size": (_SZ, [C.c_uint32, _I, _I, _I]),
+    "tss_frame": (_I, [_P, C.c_uint32, _P, _SZ, C.POINTER(_SZ)]),
+    "tss_batch_frame": (_I, [_P, _I, C.c_uint32, _P, _SZ, C.POINTER(_SZ)]),
     "tss_batch_conservation": (_I, [_P, _I, _P, _P]),
     "tss_batch_synchronize": (_I, [_P]),
     "tss_batch_num_scenes": (_I, [_P, C.POINTER(_I)]),

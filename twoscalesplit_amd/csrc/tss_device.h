@@ -196,6 +196,23 @@ void launch_work_counts(const DevState& d, const StepShape& sh,
                         unsigned long long* out, void* stream);
 void launch_fill_int(int* p, int value, long long n, void* stream);
 void launch_iota(int* p, int first, int n, void* stream);
+// Frame export sections (any pointer may be null: field not exported).
+struct FrameOut {
+    int* id;
+    float2* pos;
+    float2* vel;
+    float* rho;
+    float* p;
+    float* vis;
+    uint8_t* state;
+};
+// Tiles of the selection over a list of n entries; tile_cnt holds
+// frame_tiles(n) + 1 ints (the last one receives the selected total).
+int frame_tiles(int n);
+// The entries of list[0..n) in [lo, hi), in list order, gathered into out
+// (at most n_out entries written); enqueued on stream.
+void launch_frame_pack(const DevState& d, const int* list, int n, int lo, int hi,
+                       int* tile_cnt, int n_out, const FrameOut& out, void* stream);
 void launch_dam_break(float2* pos, int nx, int ny, float dx, float jitter,
                       unsigned long long seed, unsigned long long scene,
                       void* stream);

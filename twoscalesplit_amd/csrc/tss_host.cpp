@@ -30,6 +30,20 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// frame export layout: sections padded to 16 bytes, in field-bit order
+size_t frame_pad(size_t x) { return (x + 15) & ~(size_t)15; }
+size_t frame_level_bytes(uint32_t f, size_t n) {
+    size_t b = 0;
+    if (f & TSS_FRAME_ID) b += frame_pad(4 * n);
+    if (f & TSS_FRAME_POS) b += frame_pad(8 * n);
+    if (f & TSS_FRAME_VEL) b += frame_pad(8 * n);
+    if (f & TSS_FRAME_RHO) b += frame_pad(4 * n);
+    if (f & TSS_FRAME_P) b += frame_pad(4 * n);
+    if (f & TSS_FRAME_VIS) b += frame_pad(4 * n);
+    if (f & TSS_FRAME_STATE) b += frame_pad(n);
+    return b;
+}
+
 struct TssError : std::runtime_error {
     int code;
     TssError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -88,6 +102,7 @@ float sqrt_threshold(float R, bool inclusive) {
 constexpr double kPi = 3.14159265358979323846;  // R/WCSPHSolver.cu:19
 static_assert(TSS_PHASE_COUNT == tss::kTimingPoints - 1, "phase count");
 static_assert(TSS_WORK_COUNT == tss::kWorkCounts, "work counter count");
+static_assert(sizeof(tss_frame_header) == 64, "frame header is 64 bytes");
 
 // Bulk value (in units of rho0) of the high-level density estimator: the
 // blended kernel (W_h + W_H)/2 summed only over r <= H (R:402-474), i.e.
@@ -209,6 +224,8 @@ struct tss_batch {
     uint32_t* nb_counts = nullptr;  // neighbour inspection scratch
     uint32_t* nb_sums = nullptr;
     size_t nb_cap = 0;
+    void* frame_buf = nullptr;      // frame export staging (grown on demand)
+    size_t frame_cap = 0;
 
     ~tss_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -222,6 +239,7 @@ struct tss_batch {
         for (void* p : owned) (void)hipFree(p);
         if (nb_counts) (void)hipFree(nb_counts);
         if (nb_sums) (void)hipFree(nb_sums);
+        if (frame_buf) (void)hipFree(frame_buf);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1127,6 +1145,90 @@ struct tss_batch {
         }
     }
 
+    // ---- frame export ------------------------------------------------------
+    void frame(int sid, uint32_t fields, void* dst, size_t cap, size_t* bytes) {
+        require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
+        require((fields & ~(uint32_t)TSS_FRAME_ALL) == 0, "unknown frame field bits");
+        const SceneHost& s = scenes[sid];
+        const tss::SceneCount sc = read_scnt(sid);
+        const int nb = (fields & TSS_FRAME_BND) ? s.nb : 0;
+        const size_t size = tss_frame_size(fields, sc.n_l, sc.n_h, nb);
+        *bytes = size;
+        if (!dst) return;
+        require(cap >= size, "frame buffer too small (" + std::to_string(size) + " bytes needed)");
+        const tss::DevCounts c = read_counts();
+        const size_t lb = frame_level_bytes(fields, sc.n_l);
+        const size_t hb = frame_level_bytes(fields, sc.n_h);
+        const int tl_n = tss::frame_tiles(c.n_l) + 1, th_n = tss::frame_tiles(c.n_h) + 1;
+        const size_t need = frame_pad(lb + hb) + 4 * (size_t)(tl_n + th_n);
+        if (frame_cap < need) {
+            if (frame_buf) (void)hipFree(frame_buf);
+            frame_buf = nullptr;
+            frame_cap = 0;
+            HIP_CHECK(hipMalloc(&frame_buf, need));
+            frame_cap = need;
+        }
+        char* stage = (char*)frame_buf;
+        int* tl = (int*)(stage + frame_pad(lb + hb));
+        int* th = tl + tl_n;
+        auto sections = [&](char* base, size_t n) {
+            tss::FrameOut o{};
+            size_t off = 0;
+            auto take = [&](uint32_t bit, size_t elem) -> void* {
+                if (!(fields & bit)) return nullptr;
+                void* p = base + off;
+                off += frame_pad(elem * n);
+                return p;
+            };
+            o.id = (int*)take(TSS_FRAME_ID, 4);
+            o.pos = (float2*)take(TSS_FRAME_POS, 8);
+            o.vel = (float2*)take(TSS_FRAME_VEL, 8);
+            o.rho = (float*)take(TSS_FRAME_RHO, 4);
+            o.p = (float*)take(TSS_FRAME_P, 4);
+            o.vis = (float*)take(TSS_FRAME_VIS, 4);
+            o.state = (uint8_t*)take(TSS_FRAME_STATE, 1);
+            return o;
+        };
+        const int par = sh.list_parity;
+        const int CL = d.CL;
+        // padding bytes of the sections are zero
+        if (lb + hb) HIP_CHECK(hipMemsetAsync(stage, 0, lb + hb, stream));
+        tss::launch_frame_pack(d, d.list_l[par], c.n_l, s.lbase, s.lbase + s.lcap, tl, sc.n_l,
+                               sections(stage, sc.n_l), stream);
+        tss::launch_frame_pack(d, d.list_h[par], c.n_h, CL + 4 * s.lbase,
+                               CL + 4 * (s.lbase + s.lcap), th, sc.n_h,
+                               sections(stage + lb, sc.n_h), stream);
+        char* out = (char*)dst;
+        int tot[2] = {0, 0};
+        if (lb + hb)
+            HIP_CHECK(hipMemcpyAsync(out + sizeof(tss_frame_header), stage, lb + hb,
+                                     hipMemcpyDeviceToHost, stream));
+        if (c.n_l) HIP_CHECK(hipMemcpyAsync(&tot[0], tl + tl_n - 1, 4, hipMemcpyDeviceToHost, stream));
+        if (c.n_h) HIP_CHECK(hipMemcpyAsync(&tot[1], th + th_n - 1, 4, hipMemcpyDeviceToHost, stream));
+        float dt = 0;
+        HIP_CHECK(hipMemcpyAsync(&dt, &d.scenes[sid].dt, 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        require(tot[0] == sc.n_l && tot[1] == sc.n_h, "frame selection disagrees with the scene counts",
+                TSS_E_STATE);
+        if (nb) read_field(sid, TSS_POS_B, out + sizeof(tss_frame_header) + lb + hb, 8 * (size_t)nb);
+        tss_frame_header hd{};
+        hd.magic = TSS_FRAME_MAGIC;
+        hd.version = TSS_FRAME_VERSION;
+        hd.fields = fields;
+        hd.step = sc.step;
+        hd.n_low = sc.n_l;
+        hd.n_high = sc.n_h;
+        hd.n_bnd = nb;
+        hd.dt = dt;
+        for (int k = 0; k < 2; ++k) {
+            hd.domain_origin[k] = s.cfg.domain_origin[k];
+            hd.domain_end[k] = s.cfg.domain_end[k];
+        }
+        std::memcpy(out, &hd, sizeof(hd));
+        const size_t bpad = frame_pad(8 * (size_t)nb) - 8 * (size_t)nb;
+        if (bpad) std::memset(out + size - bpad, 0, bpad);
+    }
+
     void set_counts(int sid, int nl, int nh, int ng) {
         int alive = 0;
         for (auto& x : scenes) alive += x.alive;
@@ -1528,6 +1630,28 @@ int tss_batch_counts(tss_batch* b, int sid, tss_counts_t* o) {
     return guarded([&] {
         require(b && o, "null argument");
         b->counts(sid, o);
+    });
+}
+
+size_t tss_frame_size(uint32_t fields, int n_low, int n_high, int n_bnd) {
+    if ((fields & ~(uint32_t)TSS_FRAME_ALL) || n_low < 0 || n_high < 0 || n_bnd < 0) return 0;
+    return sizeof(tss_frame_header) + frame_level_bytes(fields, (size_t)n_low) +
+           frame_level_bytes(fields, (size_t)n_high) +
+           ((fields & TSS_FRAME_BND) ? frame_pad(8 * (size_t)n_bnd) : 0);
+}
+
+int tss_frame(tss_solver* s, uint32_t fields, void* dst, size_t cap, size_t* bytes) {
+    return guarded([&] {
+        require(s && bytes, "null argument");
+        s->b.frame(s->sid, fields, dst, cap, bytes);
+    });
+}
+
+int tss_batch_frame(tss_batch* b, int sid, uint32_t fields, void* dst, size_t cap,
+                    size_t* bytes) {
+    return guarded([&] {
+        require(b && bytes, "null argument");
+        b->frame(sid, fields, dst, cap, bytes);
     });
 }
 

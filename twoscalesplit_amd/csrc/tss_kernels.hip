@@ -1390,6 +1390,97 @@ __global__ void k_iota(int* __restrict__ p, int first, int n) {
     GRID_STRIDE(i, n) p[i] = first + i;
 }
 
+// ---------------------------------------------------------------------------
+// Frame export: the entries of a level's current list that a scene owns
+// (unified ids in [lo, hi)), in list order, gathered field by field into
+// packed sections. Stable three-kernel selection: per-tile counts, one
+// exclusive scan over the tiles, then a ballot prefix inside each tile.
+// ---------------------------------------------------------------------------
+constexpr int kFrameThreads = 256;
+constexpr int kFrameChunks = 4;
+constexpr int kFrameTile = kFrameThreads * kFrameChunks;
+
+__global__ __launch_bounds__(kFrameThreads) void k_frame_count(
+    const int* __restrict__ list, int n, int lo, int hi, int* __restrict__ cnt) {
+    __shared__ int wsum[kFrameThreads / 64];
+    int c = 0;
+    for (int k = 0; k < kFrameChunks; ++k) {
+        const int i = blockIdx.x * kFrameTile + k * kFrameThreads + threadIdx.x;
+        if (i < n) {
+            const int v = list[i];
+            c += v >= lo && v < hi;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kFrameThreads / 64; ++w) t += wsum[w];
+        cnt[blockIdx.x] = t;
+    }
+}
+
+// in-place exclusive scan of cnt[0..nt); cnt[nt] = total. One block.
+__global__ __launch_bounds__(1024) void k_frame_scan(int* __restrict__ cnt, int nt) {
+    __shared__ int part[1024];
+    __shared__ int base;
+    if (threadIdx.x == 0) base = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nt; b0 += 1024) {
+        const int i = b0 + threadIdx.x;
+        const int v = i < nt ? cnt[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int a = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (i < nt) cnt[i] = base + part[threadIdx.x] - v;
+        const int tot = part[1023];
+        __syncthreads();
+        if (threadIdx.x == 0) base += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[nt] = base;
+}
+
+__global__ __launch_bounds__(kFrameThreads) void k_frame_pack(
+    DevState d, const int* __restrict__ list, int n, int lo, int hi,
+    const int* __restrict__ offs, int n_out, FrameOut out) {
+    __shared__ int wsum[kFrameThreads / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int base = offs[blockIdx.x];
+    for (int k = 0; k < kFrameChunks; ++k) {
+        const int i = blockIdx.x * kFrameTile + k * kFrameThreads + threadIdx.x;
+        const int v = i < n ? list[i] : -1;
+        const bool keep = i < n && v >= lo && v < hi;
+        const unsigned long long m = __ballot(keep);
+        const int wpre = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int bpre = 0, tot = 0;
+        for (int q = 0; q < kFrameThreads / 64; ++q) {
+            bpre += q < w ? wsum[q] : 0;
+            tot += wsum[q];
+        }
+        const int o = base + bpre + wpre;
+        if (keep && o < n_out) {
+            if (out.id) out.id[o] = v - lo;
+            if (out.pos) out.pos[o] = d.pos[v];
+            if (out.vel) out.vel[o] = d.vel[v];
+            if (out.rho) out.rho[o] = d.rho[v];
+            if (out.p) out.p[o] = d.p[v];
+            if (out.vis) out.vis[o] = d.vis[v];
+            if (out.state) out.state[o] = d.state[v];
+        }
+        base += tot;
+        __syncthreads();
+    }
+}
+
 void launch_dam_break(float2* pos, int nx, int ny, float dx, float jitter,
                       unsigned long long seed, unsigned long long scene,
                       void* stream) {
@@ -1481,6 +1572,18 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     k_finalize<<<sh.S, 256, 0, st>>>(d);
     mark(8);
 }
+
+void launch_frame_pack(const DevState& d, const int* list, int n, int lo, int hi,
+                       int* tile_cnt, int n_out, const FrameOut& out, void* stream) {
+    if (n <= 0) return;
+    hipStream_t st = (hipStream_t)stream;
+    const int nt = (n + kFrameTile - 1) / kFrameTile;
+    k_frame_count<<<nt, kFrameThreads, 0, st>>>(list, n, lo, hi, tile_cnt);
+    k_frame_scan<<<1, 1024, 0, st>>>(tile_cnt, nt);
+    k_frame_pack<<<nt, kFrameThreads, 0, st>>>(d, list, n, lo, hi, tile_cnt, n_out, out);
+}
+
+int frame_tiles(int n) { return (n + kFrameTile - 1) / kFrameTile; }
 
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
                          void* stream) {

@@ -192,6 +192,12 @@ class Solver:
         return {"tuning": t.as_dict(), "steps_taken": taken.value,
                 "trials": log[:nlog.value].tolist(), "_struct": t}
 
+    def frame(self, fields: int = 255):
+        """The current frame (frames.Frame): list-ordered particles of both
+        levels with the requested fields (frames.ID | frames.POS | ...)."""
+        from . import frames
+        return frames.grab(self, fields)
+
     def snapshot(self) -> dict:
         """Every by-id field, the lists and dt (synchronising)."""
         cnt = self.counts()
@@ -245,6 +251,12 @@ class Batch:
         except Exception:
             pass
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
     def add_scene(self, cfg: N.TssConfig, low_xy: np.ndarray, cap_low: int,
                   bnd_xy: np.ndarray) -> int:
         low = np.ascontiguousarray(low_xy, dtype=np.float32).reshape(-1, 2)
@@ -296,6 +308,11 @@ class Batch:
                                        N.ptr(out), out.nbytes),
                 f"tss_batch_read({name})")
         return out
+
+    def frame(self, sid: int, fields: int = 255):
+        """The current frame of one scene (frames.Frame)."""
+        from . import frames
+        return frames.grab(self, fields, sid)
 
     def conservation(self, sid: int) -> tuple[np.ndarray, np.ndarray]:
         out = np.zeros(5, np.float64)
