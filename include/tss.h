@@ -356,6 +356,57 @@ int tss_frame(tss_solver* s, uint32_t fields, void* dst, size_t cap,
 int tss_batch_frame(tss_batch* b, int scene_id, uint32_t fields, void* dst,
                     size_t cap, size_t* bytes);
 
+/* ---- scene snapshot / resume ---------------------------------------------- */
+/* A self-contained image of one scene between steps: its configuration,
+ * boundary, every per-particle array of its id ranges, its lists (in list
+ * order, scene-local ids), counters, the dt of the next step and the last
+ * conservation sums. Restoring it -- into a new solver, or into any batch
+ * with room -- continues the run bit for bit as if it had not stopped
+ * (results of a scene do not depend on where its ids or cells sit).
+ * Replaces ParticleSystem::Save (R/ParticleSystem.cpp:100-107), which
+ * writes only the particle count.
+ *
+ * Layout: tss_snapshot_header (padded to a multiple of 16 bytes), then
+ * sections each padded to 16 bytes: boundary positions by id
+ * (float2[n_bnd]); low arrays over cap_low ids: pos, vel, acc (float2), rho,
+ * p, vis (float), state (uint8), age (int32); high arrays over 4*cap_low
+ * ids: pos, vel, acc, rho, p, vis, state; lists: low (int32[n_low]), high
+ * (int32[n_high]), groups (int32[n_groups]). */
+#define TSS_SNAPSHOT_MAGIC 0x53535354u  /* bytes "TSSS" */
+#define TSS_SNAPSHOT_VERSION 1u
+typedef struct tss_snapshot_header {
+    uint32_t magic, version;
+    uint32_t header_bytes;        /* sizeof(tss_snapshot_header) */
+    uint32_t config_bytes;        /* sizeof(tss_config) */
+    uint64_t total_bytes;
+    tss_config cfg;
+    int32_t cap_low, n_bnd;
+    int32_t n_low, n_high, n_groups;
+    int32_t n_fresh;              /* trailing high-list entries injected by
+                                     the last step (relaxed by the next) */
+    int32_t step, n_low_last, n_high_last;
+    int32_t n_split, n_promote, n_merge, n_dissolve, n_cand;
+    int64_t particle_steps;
+    float dt;                     /* dt of the next step */
+    int32_t reserved0;
+    double conservation[10];      /* last step's sums and error bounds */
+} tss_snapshot_header;
+/* Bytes of a snapshot with these sizes (0 for invalid arguments). */
+size_t tss_snapshot_size(int cap_low, int n_bnd, int n_low, int n_high,
+                         int n_groups);
+/* Snapshot of a scene into dst (synchronising; between steps). *bytes gets
+ * the snapshot's size; dst == NULL is a size query. TSS_E_INVALID if cap is
+ * smaller. */
+int tss_snapshot(tss_solver* s, void* dst, size_t cap, size_t* bytes);
+int tss_batch_snapshot(tss_batch* b, int scene_id, void* dst, size_t cap,
+                       size_t* bytes);
+/* A new single-scene solver holding the snapshot's scene. */
+int tss_create_from_snapshot(const void* src, size_t bytes, tss_solver** out);
+/* Adds the snapshot's scene to a batch (TSS_E_NOMEM without room). The
+ * snapshot is validated (sizes, ids, states, finite positions) first. */
+int tss_batch_restore(tss_batch* b, const void* src, size_t bytes,
+                      int* scene_id);
+
 /* ---- serving loop ---------------------------------------------------------- */
 /* A scene request: its initial state, the steps it asks for and its arrival
  * time; optional output buffers receive fields of its final state. No

@@ -9,6 +9,7 @@ import math
 import os
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 import pytest
@@ -271,11 +272,14 @@ def test_facade_demo_runs():
     exe = os.path.join(os.path.dirname(os.path.dirname(__file__)),
                        "twoscalesplit_amd", "lib", "facade_demo")
     _say("facade demo (reference class API), 2000 steps")
-    out = subprocess.run([exe, "2000"], capture_output=True, text=True,
+    state = os.path.join(tempfile.mkdtemp(), "demo.tsss")
+    out = subprocess.run([exe, "2000", state], capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 0, out.stderr
     res = json.loads(out.stdout.strip().splitlines()[-1])
     assert res["finite"] == 1 and res["n_low"] == 101 * 151
+    # SaveState / LoadState continue the run bit for bit
+    assert res["resume_equal"] == 1
     # the collapsing column crosses x = 0.5, where the reference rule splits
     assert res["n_high"] > 0 and res["n_high"] % 4 == 0
     assert 900.0 < res["rho_mean"] < 1100.0

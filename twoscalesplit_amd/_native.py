@@ -106,6 +106,27 @@ class TssDamBreak(C.Structure):
     ]
 
 
+class TssSnapshotHeader(C.Structure):
+    """tss_snapshot_header (natural C alignment, like the library's)."""
+    _fields_ = [
+        ("magic", C.c_uint32), ("version", C.c_uint32),
+        ("header_bytes", C.c_uint32), ("config_bytes", C.c_uint32),
+        ("total_bytes", C.c_uint64),
+        ("cfg", TssConfig),
+        ("cap_low", C.c_int32), ("n_bnd", C.c_int32),
+        ("n_low", C.c_int32), ("n_high", C.c_int32), ("n_groups", C.c_int32),
+        ("n_fresh", C.c_int32),
+        ("step", C.c_int32), ("n_low_last", C.c_int32),
+        ("n_high_last", C.c_int32),
+        ("n_split", C.c_int32), ("n_promote", C.c_int32),
+        ("n_merge", C.c_int32), ("n_dissolve", C.c_int32),
+        ("n_cand", C.c_int32),
+        ("particle_steps", C.c_int64),
+        ("dt", C.c_float), ("reserved0", C.c_int32),
+        ("conservation", C.c_double * 10),
+    ]
+
+
 class TssRequest(C.Structure):
     """One scene request of tss_serve."""
     _fields_ = [
@@ -190,6 +211,11 @@ PROTOTYPES = {
     "tss_batch_counts": (_I, [_P, _I, C.POINTER(TssCounts)]),
     "tss_batch_read": (_I, [_P, _I, _I, _P, _SZ]),
     "tss_frame_size": (_SZ, [C.c_uint32, _I, _I, _I]),
+    "tss_snapshot_size": (_SZ, [_I, _I, _I, _I, _I]),
+    "tss_snapshot": (_I, [_P, _P, _SZ, C.POINTER(_SZ)]),
+    "tss_batch_snapshot": (_I, [_P, _I, _P, _SZ, C.POINTER(_SZ)]),
+    "tss_create_from_snapshot": (_I, [_P, _SZ, C.POINTER(_P)]),
+    "tss_batch_restore": (_I, [_P, _P, _SZ, C.POINTER(_I)]),
     "tss_frame": (_I, [_P, C.c_uint32, _P, _SZ, C.POINTER(_SZ)]),
     "tss_batch_frame": (_I, [_P, _I, C.c_uint32, _P, _SZ, C.POINTER(_SZ)]),
     "tss_batch_conservation": (_I, [_P, _I, _P, _P]),

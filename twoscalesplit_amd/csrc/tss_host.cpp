@@ -30,6 +30,9 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// particle states (uint8): low 0 active, 1 transient parent, 2 deleted
+constexpr uint8_t kStateActive = 0, kStateDeleted = 2;
+
 // frame export layout: sections padded to 16 bytes, in field-bit order
 size_t frame_pad(size_t x) { return (x + 15) & ~(size_t)15; }
 size_t frame_level_bytes(uint32_t f, size_t n) {
@@ -103,6 +106,7 @@ constexpr double kPi = 3.14159265358979323846;  // R/WCSPHSolver.cu:19
 static_assert(TSS_PHASE_COUNT == tss::kTimingPoints - 1, "phase count");
 static_assert(TSS_WORK_COUNT == tss::kWorkCounts, "work counter count");
 static_assert(sizeof(tss_frame_header) == 64, "frame header is 64 bytes");
+static_assert(offsetof(tss_snapshot_header, cfg) == 24, "snapshot configuration offset");
 
 // Bulk value (in units of rho0) of the high-level density estimator: the
 // blended kernel (W_h + W_H)/2 summed only over r <= H (R:402-474), i.e.
@@ -1229,6 +1233,264 @@ struct tss_batch {
         if (bpad) std::memset(out + size - bpad, 0, bpad);
     }
 
+    // ---- snapshot / resume -------------------------------------------------
+    struct Slice {
+        void* dev;
+        size_t bytes;
+    };
+    // the by-id arrays of a scene, in snapshot order
+    std::vector<Slice> scene_slices(const SceneHost& s) const {
+        const size_t lb = s.lbase, lc = s.lcap;
+        const size_t hb = (size_t)d.CL + 4 * lb, hc = 4 * lc;
+        return {{d.pos + lb, 8 * lc},  {d.vel + lb, 8 * lc}, {d.acc + lb, 8 * lc},
+                {d.rho + lb, 4 * lc},  {d.p + lb, 4 * lc},   {d.vis + lb, 4 * lc},
+                {d.state + lb, lc},    {d.age + lb, 4 * lc}, {d.pos + hb, 8 * hc},
+                {d.vel + hb, 8 * hc},  {d.acc + hb, 8 * hc}, {d.rho + hb, 4 * hc},
+                {d.p + hb, 4 * hc},    {d.vis + hb, 4 * hc}, {d.state + hb, hc}};
+    }
+    static size_t snapshot_size(long long cap_low, long long nb, long long nl, long long nh,
+                                long long ng) {
+        const size_t lc = (size_t)cap_low, hc = 4 * lc;
+        size_t b = frame_pad(sizeof(tss_snapshot_header)) + frame_pad(8 * (size_t)nb);
+        for (size_t e : {8, 8, 8, 4, 4, 4, 1, 4}) b += frame_pad(e * lc);
+        for (size_t e : {8, 8, 8, 4, 4, 4, 1}) b += frame_pad(e * hc);
+        return b + frame_pad(4 * (size_t)nl) + frame_pad(4 * (size_t)nh) + frame_pad(4 * (size_t)ng);
+    }
+
+    void snapshot(int sid, void* dst, size_t cap, size_t* bytes) {
+        require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
+        const SceneHost& s = scenes[sid];
+        const tss::SceneCount sc = read_scnt(sid);
+        const size_t size = snapshot_size(s.lcap, s.nb, sc.n_l, sc.n_h, sc.n_g);
+        *bytes = size;
+        if (!dst) return;
+        require(cap >= size, "snapshot buffer too small (" + std::to_string(size) + " bytes needed)");
+        const tss::DevCounts c = read_counts();
+        const int par = sh.list_parity;
+        // the scene's high list and how many of its entries sit in the
+        // global fresh tail (the children the next step relaxes)
+        std::vector<int> gh(c.n_h), lh;
+        if (c.n_h)
+            HIP_CHECK(hipMemcpyAsync(gh.data(), d.list_h[par], 4 * (size_t)c.n_h, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        const long long h0 = (long long)d.CL + 4LL * s.lbase, h1 = h0 + 4LL * s.lcap;
+        int fresh = 0;
+        for (int i = 0; i < c.n_h; ++i)
+            if (gh[i] >= h0 && gh[i] < h1) {
+                lh.push_back((int)(gh[i] - h0));
+                fresh += i >= c.n_h - c.n_fresh;
+            }
+        require((int)lh.size() == sc.n_h, "high list disagrees with the scene counts", TSS_E_STATE);
+        char* out = (char*)dst;
+        std::memset(out, 0, size);
+        tss_snapshot_header hd{};
+        hd.magic = TSS_SNAPSHOT_MAGIC;
+        hd.version = TSS_SNAPSHOT_VERSION;
+        hd.header_bytes = sizeof(tss_snapshot_header);
+        hd.config_bytes = sizeof(tss_config);
+        hd.total_bytes = size;
+        hd.cfg = s.cfg;
+        hd.cap_low = s.lcap;
+        hd.n_bnd = s.nb;
+        hd.n_low = sc.n_l;
+        hd.n_high = sc.n_h;
+        hd.n_groups = sc.n_g;
+        hd.n_fresh = fresh;
+        hd.step = sc.step;
+        hd.n_low_last = sc.n_l_last;
+        hd.n_high_last = sc.n_h_last;
+        hd.n_split = sc.n_split;
+        hd.n_promote = sc.n_promote;
+        hd.n_merge = sc.n_merge;
+        hd.n_dissolve = sc.n_dissolve;
+        hd.n_cand = sc.cand;
+        hd.particle_steps = (int64_t)sc.psteps;
+        HIP_CHECK(hipMemcpyAsync(&hd.dt, &d.scenes[sid].dt, 4, hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(hd.conservation, d.cons + (size_t)sid * 10, sizeof(hd.conservation),
+                                 hipMemcpyDeviceToHost, stream));
+        size_t off = frame_pad(sizeof(hd));
+        const size_t off_bnd = off;
+        off += frame_pad(8 * (size_t)s.nb);
+        for (const Slice& sl : scene_slices(s)) {
+            HIP_CHECK(hipMemcpyAsync(out + off, sl.dev, sl.bytes, hipMemcpyDeviceToHost, stream));
+            off += frame_pad(sl.bytes);
+        }
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (s.nb) read_field(sid, TSS_POS_B, out + off_bnd, 8 * (size_t)s.nb);
+        if (sc.n_l) read_field(sid, TSS_LIST_L, out + off, 4 * (size_t)sc.n_l);
+        off += frame_pad(4 * (size_t)sc.n_l);
+        if (sc.n_h) std::memcpy(out + off, lh.data(), 4 * (size_t)sc.n_h);
+        off += frame_pad(4 * (size_t)sc.n_h);
+        if (sc.n_g) read_field(sid, TSS_LIST_G, out + off, 4 * (size_t)sc.n_g);
+        off += frame_pad(4 * (size_t)sc.n_g);
+        require(off == size, "snapshot layout", TSS_E_STATE);
+        std::memcpy(out, &hd, sizeof(hd));
+    }
+
+    // header checks shared by restore and create_from_snapshot
+    static const tss_snapshot_header& snapshot_header(const void* src, size_t bytes) {
+        require(src != nullptr && bytes >= sizeof(tss_snapshot_header), "snapshot too short");
+        const tss_snapshot_header& hd = *(const tss_snapshot_header*)src;
+        require(hd.magic == TSS_SNAPSHOT_MAGIC, "not a snapshot");
+        require(hd.version == TSS_SNAPSHOT_VERSION && hd.header_bytes == sizeof(tss_snapshot_header) &&
+                    hd.config_bytes == sizeof(tss_config),
+                "snapshot version or layout not supported");
+        require(hd.cap_low > 0 && hd.n_bnd >= 0 && hd.n_low >= 0 && hd.n_low <= hd.cap_low &&
+                    hd.n_high >= 0 && hd.n_high <= 4LL * hd.cap_low && hd.n_groups >= 0 &&
+                    hd.n_groups <= hd.cap_low && hd.n_fresh >= 0 && hd.n_fresh <= hd.n_high,
+                "snapshot counts out of range");
+        require(hd.total_bytes == snapshot_size(hd.cap_low, hd.n_bnd, hd.n_low, hd.n_high, hd.n_groups) &&
+                    hd.total_bytes <= bytes,
+                "snapshot size does not match its header");
+        return hd;
+    }
+
+    // section offsets of a snapshot whose header passed snapshot_header
+    struct SnapLayout {
+        size_t bnd;
+        size_t arr[15], size[15];
+        size_t ll, lh, lg;
+    };
+    static SnapLayout snapshot_layout(const tss_snapshot_header& hd) {
+        SnapLayout L{};
+        const size_t lc = (size_t)hd.cap_low, hc = 4 * lc;
+        size_t off = frame_pad(sizeof(hd));
+        L.bnd = off;
+        off += frame_pad(8 * (size_t)hd.n_bnd);
+        const size_t sizes[15] = {8 * lc, 8 * lc, 8 * lc, 4 * lc, 4 * lc, 4 * lc, lc, 4 * lc,
+                                  8 * hc, 8 * hc, 8 * hc, 4 * hc, 4 * hc, 4 * hc, hc};
+        for (int k = 0; k < 15; ++k) {
+            L.arr[k] = off;
+            L.size[k] = sizes[k];
+            off += frame_pad(sizes[k]);
+        }
+        L.ll = off;
+        off += frame_pad(4 * (size_t)hd.n_low);
+        L.lh = off;
+        off += frame_pad(4 * (size_t)hd.n_high);
+        L.lg = off;
+        return L;
+    }
+
+    // every check a snapshot must pass before anything is allocated for it
+    static const tss_snapshot_header& check_snapshot(const void* src, size_t bytes) {
+        const tss_snapshot_header& hd = snapshot_header(src, bytes);
+        validate(hd.cfg, 0, hd.cap_low, hd.n_bnd);
+        const char* in = (const char*)src;
+        const SnapLayout L = snapshot_layout(hd);
+        const size_t lc = (size_t)hd.cap_low, hc = 4 * lc;
+        const size_t* offs = L.arr;
+        const size_t* sizes = L.size;
+        const float* bnd = (const float*)(in + L.bnd);
+        for (size_t i = 0; i < 2 * (size_t)hd.n_bnd; ++i)
+            require(std::isfinite(bnd[i]), "non-finite boundary position in snapshot");
+        // per-particle arrays: finite positions and velocities, known states
+        for (int k : {0, 1, 8, 9}) {
+            const float* f = (const float*)(in + offs[k]);
+            for (size_t i = 0; i < sizes[k] / 4; ++i)
+                require(std::isfinite(f[i]), "non-finite position or velocity in snapshot");
+        }
+        for (int k : {6, 14}) {
+            const uint8_t* st = (const uint8_t*)(in + offs[k]);
+            for (size_t i = 0; i < sizes[k]; ++i) require(st[i] <= 2, "unknown particle state in snapshot");
+        }
+        // lists: ids in range, no repeats
+        const int* ll = (const int*)(in + L.ll);
+        const int* lh = (const int*)(in + L.lh);
+        const int* lg = (const int*)(in + L.lg);
+        auto check_list = [](const int* l, int n, size_t range, const char* what) {
+            std::vector<uint8_t> seen(range, 0);
+            for (int i = 0; i < n; ++i) {
+                require(l[i] >= 0 && (size_t)l[i] < range && !seen[l[i]],
+                        std::string("invalid ") + what + " list in snapshot");
+                seen[l[i]] = 1;
+            }
+        };
+        check_list(ll, hd.n_low, lc, "low");
+        check_list(lh, hd.n_high, hc, "high");
+        check_list(lg, hd.n_groups, lc, "group");
+        // the list invariants that bound every later step's list growth:
+        // listed low particles are active or transient, group parents are
+        // transient or deleted (so they cannot split again), and every
+        // listed child belongs to a listed group
+        const uint8_t* st_l = (const uint8_t*)(in + offs[6]);
+        std::vector<uint8_t> group(lc, 0);
+        for (int i = 0; i < hd.n_groups; ++i) {
+            require(st_l[lg[i]] != kStateActive, "group parent is active in snapshot");
+            group[lg[i]] = 1;
+        }
+        for (int i = 0; i < hd.n_low; ++i)
+            require(st_l[ll[i]] != kStateDeleted, "deleted particle listed in snapshot");
+        for (int i = 0; i < hd.n_high; ++i)
+            require(group[lh[i] >> 2], "child of an unlisted group in snapshot");
+        return hd;
+    }
+
+    int restore(const void* src, size_t bytes) {
+        const tss_snapshot_header& hd = check_snapshot(src, bytes);
+        const char* in = (const char*)src;
+        const SnapLayout L = snapshot_layout(hd);
+        const size_t* offs = L.arr;
+        const size_t* sizes = L.size;
+        const float* bnd = (const float*)(in + L.bnd);
+        const int* ll = (const int*)(in + L.ll);
+        const int* lh = (const int*)(in + L.lh);
+        const int* lg = (const int*)(in + L.lg);
+        const int sid = add_scene(hd.cfg, nullptr, 0, hd.cap_low, bnd, hd.n_bnd);
+        const SceneHost& s = scenes[sid];
+        const std::vector<Slice> sl = scene_slices(s);
+        for (int k = 0; k < 15; ++k)
+            HIP_CHECK(hipMemcpyAsync(sl[k].dev, in + offs[k], sizes[k], hipMemcpyHostToDevice, stream));
+        // lists in unified ids; the scene's kept high entries go before the
+        // batch's fresh tail and its own fresh entries after it, so every
+        // scene's fresh children stay at the end of the high list
+        tss::DevCounts c = read_counts();
+        const int par = sh.list_parity;
+        const int CL = d.CL;
+        std::vector<int> ul(ll, ll + hd.n_low), ug(lg, lg + hd.n_groups);
+        for (int& x : ul) x += s.lbase;
+        for (int& x : ug) x += s.lbase;
+        const int kept = hd.n_high - hd.n_fresh;
+        std::vector<int> uh((size_t)hd.n_high + c.n_fresh);
+        for (int i = 0; i < kept; ++i) uh[i] = lh[i] + CL + 4 * s.lbase;
+        const int t0 = c.n_h - c.n_fresh;
+        if (c.n_fresh)
+            HIP_CHECK(hipMemcpyAsync(uh.data() + kept, d.list_h[par] + t0, 4 * (size_t)c.n_fresh,
+                                     hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (int i = kept; i < hd.n_high; ++i) uh[c.n_fresh + i] = lh[i] + CL + 4 * s.lbase;
+        if (hd.n_low)
+            HIP_CHECK(hipMemcpyAsync(d.list_l[par] + c.n_l, ul.data(), 4 * ul.size(), hipMemcpyHostToDevice, stream));
+        if (hd.n_groups)
+            HIP_CHECK(hipMemcpyAsync(d.list_g[par] + c.n_g, ug.data(), 4 * ug.size(), hipMemcpyHostToDevice, stream));
+        if (!uh.empty())
+            HIP_CHECK(hipMemcpyAsync(d.list_h[par] + t0, uh.data(), 4 * uh.size(), hipMemcpyHostToDevice, stream));
+        c.n_l += hd.n_low;
+        c.n_h += hd.n_high;
+        c.n_g += hd.n_groups;
+        c.n_fresh += hd.n_fresh;
+        tss::SceneCount sc{};
+        sc.n_l = hd.n_low;
+        sc.n_h = hd.n_high;
+        sc.n_g = hd.n_groups;
+        sc.n_l_last = hd.n_low_last;
+        sc.n_h_last = hd.n_high_last;
+        sc.n_split = hd.n_split;
+        sc.n_promote = hd.n_promote;
+        sc.n_merge = hd.n_merge;
+        sc.n_dissolve = hd.n_dissolve;
+        sc.cand = hd.n_cand;
+        sc.step = hd.step;
+        sc.psteps = (unsigned long long)hd.particle_steps;
+        HIP_CHECK(hipMemcpyAsync(d.scnt + sid, &sc, sizeof(sc), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(&d.scenes[sid].dt, &hd.dt, 4, hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d.cons + (size_t)sid * 10, hd.conservation, sizeof(hd.conservation),
+                                 hipMemcpyHostToDevice, stream));
+        write_counts(c);
+        invalidate_graphs();
+        return sid;
+    }
+
     void set_counts(int sid, int nl, int nh, int ng) {
         int alive = 0;
         for (auto& x : scenes) alive += x.alive;
@@ -1652,6 +1914,52 @@ int tss_batch_frame(tss_batch* b, int sid, uint32_t fields, void* dst, size_t ca
     return guarded([&] {
         require(b && bytes, "null argument");
         b->frame(sid, fields, dst, cap, bytes);
+    });
+}
+
+size_t tss_snapshot_size(int cap_low, int n_bnd, int n_low, int n_high, int n_groups) {
+    if (cap_low <= 0 || n_bnd < 0 || n_low < 0 || n_low > cap_low || n_high < 0 ||
+        (long long)n_high > 4LL * cap_low || n_groups < 0 || n_groups > cap_low)
+        return 0;
+    return tss_batch::snapshot_size(cap_low, n_bnd, n_low, n_high, n_groups);
+}
+
+int tss_snapshot(tss_solver* s, void* dst, size_t cap, size_t* bytes) {
+    return guarded([&] {
+        require(s && bytes, "null argument");
+        s->b.snapshot(s->sid, dst, cap, bytes);
+    });
+}
+
+int tss_batch_snapshot(tss_batch* b, int sid, void* dst, size_t cap, size_t* bytes) {
+    return guarded([&] {
+        require(b && bytes, "null argument");
+        b->snapshot(sid, dst, cap, bytes);
+    });
+}
+
+int tss_batch_restore(tss_batch* b, const void* src, size_t bytes, int* scene_id) {
+    return guarded([&] {
+        require(b && scene_id, "null argument");
+        *scene_id = b->restore(src, bytes);
+    });
+}
+
+int tss_create_from_snapshot(const void* src, size_t bytes, tss_solver** out) {
+    return guarded([&] {
+        require(out != nullptr, "null argument");
+        *out = nullptr;
+        const tss_snapshot_header& hd = tss_batch::check_snapshot(src, bytes);
+        auto s = std::make_unique<tss_solver>();
+        tss_batch_desc ds{};
+        ds.max_scenes = 1;
+        ds.cap_low = hd.cap_low;
+        ds.cap_bnd = hd.n_bnd;
+        ds.cells_low = (long long)hd.cfg.domain_dims[0] * hd.cfg.domain_dims[1];
+        ds.cells_high = (long long)hd.cfg.domain_dims_high[0] * hd.cfg.domain_dims_high[1];
+        s->b.create(ds);
+        s->sid = s->b.restore(src, bytes);
+        *out = s.release();
     });
 }
 

@@ -17,6 +17,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -77,6 +78,16 @@ public:
 
     void Map();
     void Unmap() { mIsMapped = false; }
+
+    // R/ParticleSystem.cpp:100-107: the particle count as a 4-byte file
+    // (WCSPHSolver::SaveState keeps the whole scene)
+    void Save(const std::string& filename) const {
+        std::FILE* f = std::fopen(filename.c_str(), "wb");
+        if (!f) throw std::runtime_error("cannot open " + filename);
+        const size_t ok = std::fwrite(&mNumParticles, sizeof(unsigned int), 1, f);
+        std::fclose(f);
+        if (ok != 1) throw std::runtime_error("cannot write " + filename);
+    }
 
 private:
     void requireMapped() const {
@@ -184,6 +195,45 @@ public:
     void AdvanceHigh() { AdvanceTS(); }
 
     tss_solver* Handle() const { return mSolver; }
+
+    // The whole scene (tss_snapshot) to a file, and back: LoadState replaces
+    // the solver's scene, and stepping continues the saved run bit for bit.
+    void SaveState(const std::string& filename) const {
+        size_t n = 0;
+        tss_detail::check(tss_snapshot(mSolver, nullptr, 0, &n), "SaveState");
+        std::vector<uint8_t> buf(n);
+        tss_detail::check(tss_snapshot(mSolver, buf.data(), n, &n), "SaveState");
+        std::FILE* f = std::fopen(filename.c_str(), "wb");
+        if (!f) throw std::runtime_error("cannot open " + filename);
+        const size_t ok = std::fwrite(buf.data(), 1, n, f);
+        std::fclose(f);
+        if (ok != n) throw std::runtime_error("cannot write " + filename);
+    }
+    void LoadState(const std::string& filename) {
+        std::FILE* f = std::fopen(filename.c_str(), "rb");
+        if (!f) throw std::runtime_error("cannot open " + filename);
+        std::vector<uint8_t> buf;
+        uint8_t chunk[1 << 16];
+        size_t got;
+        while ((got = std::fread(chunk, 1, sizeof(chunk), f)) > 0)
+            buf.insert(buf.end(), chunk, chunk + got);
+        std::fclose(f);
+        tss_solver* fresh = nullptr;
+        tss_detail::check(tss_create_from_snapshot(buf.data(), buf.size(), &fresh), "LoadState");
+        tss_destroy(mSolver);
+        mSolver = fresh;
+    }
+    // The current frame (tss_frame: list-ordered ids, positions and visual
+    // quantities by default) appended to a file; returns its size.
+    size_t WriteFrame(std::FILE* f,
+                      uint32_t fields = TSS_FRAME_ID | TSS_FRAME_POS | TSS_FRAME_VIS) const {
+        size_t n = 0;
+        tss_detail::check(tss_frame(mSolver, fields, nullptr, 0, &n), "WriteFrame");
+        std::vector<uint8_t> buf(n);
+        tss_detail::check(tss_frame(mSolver, fields, buf.data(), n, &n), "WriteFrame");
+        if (std::fwrite(buf.data(), 1, n, f) != n) throw std::runtime_error("cannot write frame");
+        return n;
+    }
 
     // Read the current state of one level into its ParticleSystem.
     void ReadBack(ParticleSystem& ps) const {

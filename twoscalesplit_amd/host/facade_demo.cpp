@@ -3,11 +3,16 @@
 // by WCSPHSolver.h, headless: builds the reference's demo scene, advances
 // it and prints a one-line JSON summary of the read-back state.
 //
-// usage: facade_demo [steps]
+// usage: facade_demo [steps] [state-file]
+// With a state file it also saves the scene there, advances 20 steps,
+// reloads it, advances the same 20 steps and reports whether both runs
+// ended on the same positions ("resume_equal").
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <memory>
+#include <vector>
 
 #include "WCSPHSolver.h"
 
@@ -57,9 +62,29 @@ int main(int argc, char** argv) {
         rho_mean /= nl ? nl : 1;
         low->Unmap();
         high->Unmap();
+        int resume_equal = -1;
+        if (argc > 2) {
+            auto run20 = [&]() {
+                for (int i = 0; i < 20; ++i) solver.AdvanceTS();
+                low->Map();
+                high->Map();
+                std::vector<float> p(low->Positions(),
+                                     low->Positions() + 2 * low->GetMaxParticles());
+                p.insert(p.end(), high->Positions(),
+                         high->Positions() + 2 * high->GetMaxParticles());
+                low->Unmap();
+                high->Unmap();
+                return p;
+            };
+            solver.SaveState(argv[2]);
+            const std::vector<float> a = run20();
+            solver.LoadState(argv[2]);
+            const std::vector<float> b = run20();
+            resume_equal = std::memcmp(a.data(), b.data(), a.size() * sizeof(float)) == 0;
+        }
         std::printf("{\"steps\": %d, \"n_low\": %u, \"n_high\": %u, \"rho_mean\": %.6f, "
-                    "\"y_min\": %.6f, \"y_max\": %.6f, \"finite\": %d}\n",
-                    steps, nl, nh, rho_mean, ymin, ymax, finite);
+                    "\"y_min\": %.6f, \"y_max\": %.6f, \"finite\": %d, \"resume_equal\": %d}\n",
+                    steps, nl, nh, rho_mean, ymin, ymax, finite, resume_equal);
         solver.Unbind();
     } catch (const std::runtime_error& e) {
         std::fprintf(stderr, "error: %s\n", e.what());

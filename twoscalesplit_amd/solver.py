@@ -66,6 +66,29 @@ def config_init(xs, ys, xe, ye, h, H, rho0, B, c, alpha, sigma, dt
     return cfg
 
 
+# tss_snapshot_header: magic, version, header_bytes, config_bytes (4 x u32),
+# total_bytes (u64), then the configuration
+_CFG_OFFSET = 24
+
+
+def _snapshot_buffer(snap) -> np.ndarray:
+    if isinstance(snap, str):
+        return np.fromfile(snap, np.uint8)
+    if isinstance(snap, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bytes(snap), np.uint8).copy()
+    return np.ascontiguousarray(snap, np.uint8)
+
+
+def _take_snapshot(call) -> np.ndarray:
+    need = C.c_size_t()
+    N.check(call(None, 0, C.byref(need)), "snapshot size")
+    buf = np.zeros(need.value, np.uint8)
+    got = C.c_size_t()
+    N.check(call(buf.ctypes.data, buf.nbytes, C.byref(got)), "snapshot")
+    assert got.value == buf.nbytes
+    return buf
+
+
 class Solver:
     """One scene resident on the GPU."""
 
@@ -79,6 +102,33 @@ class Solver:
                                N.ptr(bnd), len(bnd), C.byref(h)), "tss_create")
         self._h = h
         self.cfg = cfg
+
+    @classmethod
+    def from_snapshot(cls, snap) -> "Solver":
+        """A solver holding the scene of a snapshot (bytes, a uint8 array or
+        a file path written by ``save``); stepping it continues the
+        snapshotted run bit for bit."""
+        buf = _snapshot_buffer(snap)
+        h = C.c_void_p()
+        N.check(N.lib().tss_create_from_snapshot(buf.ctypes.data, buf.nbytes,
+                                                 C.byref(h)),
+                "tss_create_from_snapshot")
+        self = cls.__new__(cls)
+        self._h = h
+        self.cfg = N.TssConfig.from_buffer_copy(
+            buf[_CFG_OFFSET:_CFG_OFFSET + C.sizeof(N.TssConfig)].tobytes())
+        return self
+
+    def save_state(self) -> np.ndarray:
+        """Snapshot of the scene (tss_snapshot) as a uint8 array."""
+        return _take_snapshot(lambda p, n, out: N.lib().tss_snapshot(
+            self._h, p, n, out))
+
+    def save(self, path: str) -> int:
+        """Writes the snapshot to a file; returns its size."""
+        s = self.save_state()
+        s.tofile(path)
+        return s.nbytes
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -308,6 +358,19 @@ class Batch:
                                        N.ptr(out), out.nbytes),
                 f"tss_batch_read({name})")
         return out
+
+    def snapshot(self, sid: int) -> np.ndarray:
+        """Snapshot of one scene (tss_batch_snapshot) as a uint8 array."""
+        return _take_snapshot(lambda p, n, out: N.lib().tss_batch_snapshot(
+            self._h, sid, p, n, out))
+
+    def restore(self, snap) -> int:
+        """Adds the scene of a snapshot to the batch; returns its id."""
+        buf = _snapshot_buffer(snap)
+        sid = C.c_int()
+        N.check(N.lib().tss_batch_restore(self._h, buf.ctypes.data, buf.nbytes,
+                                          C.byref(sid)), "tss_batch_restore")
+        return sid.value
 
     def frame(self, sid: int, fields: int = 255):
         """The current frame of one scene (frames.Frame)."""
