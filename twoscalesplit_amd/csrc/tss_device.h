@@ -98,6 +98,10 @@ struct DevCounts {
 };
 
 // Device buffers of a batch (all pointers are device memory).
+// 32-bit words of a target's candidate-mask record (up to 256 candidates of
+// its same-level scan rows; targets with more take the scan path).
+constexpr int kMaskWords = 8;
+
 struct DevState {
     int CL;                   // low capacity (unified index space size 5 CL)
     int NB;                   // boundary capacity
@@ -135,6 +139,10 @@ struct DevState {
     uint8_t* ssurf;           // surface flag (sorted)
     int* sscene;
     int* sidx;                // [5 CL] sorted index of unified id
+    // [5 CL * kMaskWords] per cell-sorted target: the same-level
+    // candidates its density pass accepted, one bit per candidate of its
+    // scan rows (each row starting on a word), read by the force pass
+    uint32_t* nmask;
     uint8_t* cand;            // [CL] split candidate flag per sorted low index
     uint8_t* merged;          // [CL] merged flag per low id
     int* tile_sums;           // [2 * max_scan_tiles]

@@ -259,6 +259,7 @@ struct tss_batch {
         b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 8;
         b += ds.cap_bnd * 12;
         b += U * (8 + 16 + 8 + 1 + 1 + 4 + 4);  // sorted copies + sidx
+        b += U * 4 * tss::kMaskWords;              // candidate masks
         b += CL * 2;
         b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) + 10 * 8 + 12);
         b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
@@ -322,6 +323,7 @@ struct tss_batch {
         d.ssurf = dalloc<uint8_t>(U, owned);
         d.sscene = dalloc<int>(U, owned);
         d.sidx = dalloc<int>(U, owned);
+        d.nmask = dalloc<uint32_t>((size_t)U * tss::kMaskWords, owned);
         d.cand = dalloc<uint8_t>(CL, owned);
         d.merged = dalloc<uint8_t>(CL, owned);
         sh.scan_tiles_l = (int)((CL + tss::kScanTile - 1) / tss::kScanTile);
