@@ -15,6 +15,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <type_traits>
 
 #include "tss_device.h"
 
