@@ -128,6 +128,7 @@ struct DevState {
     uint32_t* digit_tot;      // [kMaxPasses * 256]
     int* cstart;
     int* cend;                // [CCL + CCH]
+    int* cscene;              // [CCL + CCH] scene owning each cell
     int* bstart;
     int* bend;                // [CCL]
     float2* bpos;             // [NB] boundary positions in cell order

@@ -264,6 +264,7 @@ struct tss_batch {
         b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) + 10 * 8 + 12);
         b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
         b += 2 * (ds.cells_low + ds.cells_high);  // cell state flags
+        b += 4 * (ds.cells_low + ds.cells_high);  // cell -> scene
         b += 16 * 4 * CL;                         // relaxation scratch (when used)
         return b;
     }
@@ -311,6 +312,7 @@ struct tss_batch {
         d.hist = dalloc<uint32_t>((size_t)256 * sh.sort_tiles, owned);
         d.digit_tot = dalloc<uint32_t>(tss::kMaxPasses * 256, owned);
         d.cstart = dalloc<int>(ccl + cch, owned);
+        d.cscene = dalloc<int>(ccl + cch, owned);
         d.cend = dalloc<int>(ccl + cch, owned);
         d.bstart = dalloc<int>(ccl, owned);
         d.bend = dalloc<int>(ccl, owned);
@@ -639,6 +641,8 @@ struct tss_batch {
         HIP_CHECK(hipMemsetAsync(d.vis + hb, 0, sizeof(float) * hn, stream));
         HIP_CHECK(hipMemsetAsync(d.state + hb, 1, hn, stream));
         tss::launch_fill_int(d.scene_of + lbase, sid, cap_low, stream);
+        tss::launch_fill_int(d.cscene + sh_.cbl, sid, ncl, stream);
+        tss::launch_fill_int(d.cscene + sh_.cbh, sid, nch, stream);
         // boundary grid on the low grid (initBoundaries, R:3205-3238): iota
         // ids, hash, stable sort, cell ranges
         std::vector<int> bh(n_bnd), order(n_bnd);

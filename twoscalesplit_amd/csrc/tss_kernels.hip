@@ -403,22 +403,15 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
         d.sxv[k] = make_float4(x.x, x.y, v.x, v.y);
         d.sstate[k] = d.state[u];
         d.sidx[u] = k;
-        const int s = d.scene_of[low_of(d, u)];
+        // the scene owns the cell (cells of a scene are one range)
+        const int s = d.cscene[key];
         d.sscene[k] = s;
         // scene segments per level: low part [0, nl), high part [nl, n)
         const bool high = k >= nl;
         const int first = high ? nl : 0;
         const int last = high ? n - 1 : nl - 1;
-        bool starts = (k == first);
-        if (!starts) {
-            int up = (int)vals[k - 1];
-            starts = d.scene_of[low_of(d, up)] != s;
-        }
-        bool ends = (k == last);
-        if (!ends) {
-            int un = (int)vals[k + 1];
-            ends = d.scene_of[low_of(d, un)] != s;
-        }
+        const bool starts = k == first || d.cscene[keys[k - 1]] != s;
+        const bool ends = k == last || d.cscene[keys[k + 1]] != s;
         if (starts) {
             if (high) d.scnt[s].seg_h0 = k; else d.scnt[s].seg_l0 = k;
         }
@@ -879,39 +872,40 @@ __device__ __forceinline__ void tile_seg(const StepShape& sh, int tile,
     }
 }
 
-__device__ __forceinline__ int2 block_excl_scan2(int a, int b, int2* sh) {
-    const int t = threadIdx.x;
-    sh[t] = make_int2(a, b);
-    __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        int2 v = t >= off ? sh[t - off] : make_int2(0, 0);
-        __syncthreads();
-        sh[t].x += v.x;
-        sh[t].y += v.y;
-        __syncthreads();
-    }
-    int2 r = make_int2(sh[t].x - a, sh[t].y - b);
-    return r;
-}
+// A compaction tile is kScanItems chunks of kScanThreads consecutive
+// entries; chunk q of tile t holds entries t * kScanTile + q * kScanThreads
+// + thread (coalesced), and entries are ranked in index order with wave
+// ballots.
+constexpr int kScanWaves = kScanThreads / 64;
 
 __global__ __launch_bounds__(kScanThreads) void k_compact_reduce(
     DevState d, StepShape sh, const uint32_t* __restrict__ vals,
     const int* __restrict__ list_g) {
-    __shared__ int2 part[kScanThreads];
+    __shared__ int wsum[2][kScanWaves];
     const int nl = d.counts->n_l, nh = d.counts->n_h, ng = d.counts->n_g;
     int seg, t;
     tile_seg(sh, blockIdx.x, seg, t);
-    int a = 0, b = 0;
-    const int i0 = t * kScanTile + threadIdx.x * kScanItems;
+    int a = 0, b = 0;  // wave totals (wave-uniform)
+    const int i0 = t * kScanTile + (int)threadIdx.x;
     for (int q = 0; q < kScanItems; ++q) {
-        Flags f = entry_flags(d, vals, list_g, seg, i0 + q, nl, nh, ng);
-        a += f.p0;
-        b += f.p1;
+        const Flags f = entry_flags(d, vals, list_g, seg, i0 + q * kScanThreads, nl, nh, ng);
+        a += __popcll(__ballot(f.p0));
+        b += __popcll(__ballot(f.p1));
     }
-    int2 ex = block_excl_scan2(a, b, part);
-    if (threadIdx.x == kScanThreads - 1) {
-        d.tile_sums[2 * blockIdx.x] = ex.x + a;
-        d.tile_sums[2 * blockIdx.x + 1] = ex.y + b;
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][w] = a;
+        wsum[1][w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ta = 0, tb = 0;
+        for (int v = 0; v < kScanWaves; ++v) {
+            ta += wsum[0][v];
+            tb += wsum[1][v];
+        }
+        d.tile_sums[2 * blockIdx.x] = ta;
+        d.tile_sums[2 * blockIdx.x + 1] = tb;
     }
 }
 
@@ -978,39 +972,55 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
     DevState d, StepShape sh, const uint32_t* __restrict__ vals,
     const int* __restrict__ list_g, int* __restrict__ nlist_l,
     int* __restrict__ nlist_h, int* __restrict__ nlist_g) {
-    __shared__ int2 part[kScanThreads];
+    __shared__ int wsum[2][kScanWaves];
     const int nl = d.counts->n_l, nh = d.counts->n_h, ng = d.counts->n_g;
     int seg, t;
     tile_seg(sh, blockIdx.x, seg, t);
-    Flags f[kScanItems];
-    int a = 0, b = 0;
-    const int i0 = t * kScanTile + threadIdx.x * kScanItems;
-    for (int q = 0; q < kScanItems; ++q) {
-        f[q] = entry_flags(d, vals, list_g, seg, i0 + q, nl, nh, ng);
-        a += f[q].p0;
-        b += f[q].p1;
-    }
-    int2 ex = block_excl_scan2(a, b, part);
-    int g0 = d.tile_base[2 * blockIdx.x] + ex.x;
-    int g1 = d.tile_base[2 * blockIdx.x + 1] + ex.y;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // output positions of the tile's first entry
+    int run0 = d.tile_base[2 * blockIdx.x];
+    int run1 = d.tile_base[2 * blockIdx.x + 1];
     const int keepL = d.seg_base[3], keepH = d.seg_base[4],
               keepG = d.seg_base[5];
     for (int q = 0; q < kScanItems; ++q) {
-        const int i = i0 + q;
+        const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
+        const Flags fq = entry_flags(d, vals, list_g, seg, i, nl, nh, ng);
+        const uint64_t m0 = __ballot(fq.p0), m1 = __ballot(fq.p1);
+        if (lane == 0) {
+            wsum[0][wv] = __popcll(m0);
+            wsum[1][wv] = __popcll(m1);
+        }
+        __syncthreads();
+        int pre0 = 0, pre1 = 0, tot0 = 0, tot1 = 0;
+        for (int v = 0; v < kScanWaves; ++v) {
+            const int c0 = wsum[0][v], c1 = wsum[1][v];
+            pre0 += v < wv ? c0 : 0;
+            pre1 += v < wv ? c1 : 0;
+            tot0 += c0;
+            tot1 += c1;
+        }
+        __syncthreads();  // wsum is rewritten by the next chunk
+        // exclusive output positions of this entry
+        const int g0 = run0 + pre0 + __popcll(m0 & lt);
+        const int g1 = run1 + pre1 + __popcll(m1 & lt);
+        run0 += tot0;
+        run1 += tot1;
         if (seg == 0 && i < nl) {
             const int u = (int)vals[i];
             const int s = d.sscene[i];
             SceneCount& sct = d.scnt[s];
-            if (i == sct.seg_l0) {
+            // the scene's first / last sorted low entry
+            if (i == 0 || d.sscene[i - 1] != s) {
                 sct.kl0 = g0;
                 sct.sp0 = g1;
             }
-            if (i == sct.seg_l1 - 1) {
-                sct.kl1 = g0 + f[q].p0;
-                sct.sp1 = g1 + f[q].p1;
+            if (i == nl - 1 || d.sscene[i + 1] != s) {
+                sct.kl1 = g0 + fq.p0;
+                sct.sp1 = g1 + fq.p1;
             }
-            if (f[q].p0) nlist_l[g0] = u;
-            if (f[q].p1) {
+            if (fq.p0) nlist_l[g0] = u;
+            if (fq.p1) {
                 // split (R:956-1005, 1834-1863): children at x' +-(rc, rc)
                 const SceneDev& sc = d.scenes[s];
                 nlist_g[keepG + g1] = u;
@@ -1037,18 +1047,18 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
             }
         } else if (seg == 1 && i < nh) {
             const int k = nl + i;
-            SceneCount& sct = d.scnt[d.sscene[k]];
+            const int s = d.sscene[k];
+            SceneCount& sct = d.scnt[s];
             const int gh = g0 - d.seg_base[0];
-            if (k == sct.seg_h0) sct.kh0 = gh;
-            if (k == sct.seg_h1 - 1) sct.kh1 = gh + f[q].p0;
-            if (f[q].p0) nlist_h[gh] = (int)vals[k];
+            // the scene's first / last sorted high entry
+            if (k == nl || d.sscene[k - 1] != s) sct.kh0 = gh;
+            if (k == nl + nh - 1 || d.sscene[k + 1] != s) sct.kh1 = gh + fq.p0;
+            if (fq.p0) nlist_h[gh] = (int)vals[k];
         } else if (seg == 2 && i < ng) {
             const int p = list_g[i];
-            if (f[q].p0) nlist_g[g0 - d.seg_base[1]] = p;
-            if (f[q].p1) nlist_l[keepL + (g1 - d.seg_base[2])] = p;
+            if (fq.p0) nlist_g[g0 - d.seg_base[1]] = p;
+            if (fq.p1) nlist_l[keepL + (g1 - d.seg_base[2])] = p;
         }
-        g0 += f[q].p0;
-        g1 += f[q].p1;
     }
 }
 
