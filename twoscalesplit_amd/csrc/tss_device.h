@@ -135,6 +135,7 @@ struct DevState {
     int* bids;                // [NB] boundary ids (scene-local) in cell order
     float2* spos;
     float4* sxv;              // [5 CL] cell-sorted (x, y, vx, vy)
+    float4* snew;             // [5 CL] cell-sorted state after the step
     float2* srp;              // [5 CL] cell-sorted (rho, p / rho^2)
     uint8_t* sstate;
     uint8_t* ssurf;           // surface flag (sorted)

@@ -265,10 +265,24 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(
     h[threadIdx.x] = 0;
     __syncthreads();
     const int base = tile * kSortTile;
+    const int lane = threadIdx.x & 63;
     if (base < n) {
+        // keys arrive nearly sorted (lists are kept in cell order), so
+        // equal digits come in runs along the lanes: one LDS atomic per
+        // run, from the run's first lane (valid lanes are a prefix)
         for (int i = 0; i < kSortItems; ++i) {
-            int e = base + i * kSortThreads + threadIdx.x;
-            if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+            const int e = base + i * kSortThreads + threadIdx.x;
+            const bool valid = e < n;
+            const uint32_t dg = valid ? (keys[e] >> shift) & 255u : 256u;
+            const uint32_t prev = (uint32_t)__shfl_up((int)dg, 1, 64);
+            const bool start = valid && (lane == 0 || prev != dg);
+            const uint64_t starts = __ballot(start);
+            const int nvalid = __popcll(__ballot(valid));
+            if (start) {
+                const uint64_t later = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+                const int next = later ? __ffsll((long long)later) - 1 : nvalid;
+                atomicAdd(&h[dg], (uint32_t)(next - lane));
+            }
         }
     }
     __syncthreads();
@@ -595,16 +609,16 @@ __global__ __launch_bounds__(256) void k_reduce(DevState d,
         const long long i1 = i0 + kRedChunk < m ? i0 + kRedChunk : (long long)m;
         const double g = (double)sc.g;
         for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            int k = i < nl_s ? c.seg_l0 + (int)i : c.seg_h0 + (int)(i - nl_s);
-            int u = (int)vals[k];
-            bool high = u >= d.CL;
-            float2 v = d.vel[u];
-            double vx = v.x, vy = v.y;
+            const bool high = i >= nl_s;
+            int k = !high ? c.seg_l0 + (int)i : c.seg_h0 + (int)(i - nl_s);
+            // post-step position and velocity (the force pass's sorted copy)
+            const float4 xv = d.snew[k];
+            double vx = xv.z, vy = xv.w;
             double v2 = vx * vx + vy * vy;
             acc[5] = fmax(acc[5], v2);
             if (!high || d.sstate[k] == 0) {
                 double mm = high ? (double)sc.m_h : (double)sc.m_l;
-                double y = (double)d.pos[u].y;
+                double y = (double)xv.y;
                 double ke = 0.5 * (mm * v2);
                 double pe = (mm * g) * y;
                 acc[0] += mm;
