@@ -253,16 +253,20 @@ def main() -> int:
         lat.append((time.perf_counter() - a) * 1e3)
     lat = np.array(lat)
 
-    # algorithmic work per launch: mean of the counts before and after
+    # algorithmic work per launch: mean of the counts before and after.
+    # The force passes do not test their same-level candidates again: they
+    # walk the density pass's recorded accept masks, testing only the
+    # decoded candidates (the density's same-level pairs, counted here with
+    # its cross-level ones as an upper bound) for the open ball.
     work = {k: (w0[k] + w1[k]) / 2.0 for k in w0}
     fl = {
         "density_low": W_CAND * (work["cand_ll"] + work["cand_lh"]) +
                        W_DENS * work["dens_pairs_l"],
         "density_high": W_CAND * (work["cand_hh"] + work["cand_hl"]) +
                         W_DENS * work["dens_pairs_h"],
-        "force_low": W_CAND * (work["cand_ll"] + work["cand_lh"]) +
+        "force_low": W_CAND * (work["dens_pairs_l"] + work["cand_lh"]) +
                      W_FORCE * work["force_pairs_l"] + W_WALL * work["wall_l"],
-        "force_high": W_CAND * (work["cand_hh"] + work["cand_hl"]) +
+        "force_high": W_CAND * (work["dens_pairs_h"] + work["cand_hl"]) +
                       W_FORCE * work["force_pairs_h"] + W_WALL * work["wall_h"],
     }
     tflops = {k: fl[k] / (ph[k] * 1e-3) / 1e12 for k in fl if ph[k] > 0}
