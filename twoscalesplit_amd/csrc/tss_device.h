@@ -156,6 +156,7 @@ struct DevState {
     int* red_n;               // [S] chunk slots of each scene
     double* cons;             // [S * 10] conservation sums + bounds
     uint8_t* cflag;           // [2 (CCL + CCH)] cflag[2c + s] = 1: state s in cell c
+    uint8_t* csurf;           // [CCL + CCH] 1: a free-surface particle in cell c
     float4* rtmp;             // [4 CL] relaxation scratch (x', v') per target;
                               // allocated when a scene enables relaxation
 };

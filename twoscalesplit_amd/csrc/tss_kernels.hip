@@ -205,6 +205,7 @@ __global__ void k_step_begin(DevState d, long long ncells) {
         d.cend[i] = -1;
         d.cflag[2 * i] = 0;
         d.cflag[2 * i + 1] = 0;
+        d.csurf[i] = 0;
     }
     GRID_STRIDE(s, d.S) {
         SceneCount& c = d.scnt[s];
@@ -657,22 +658,24 @@ __global__ __launch_bounds__(256) void k_reduce(DevState d,
 // ---------------------------------------------------------------------------
 // Any surface particle within sqrt-distance r of x (r2 <= t)? Scans the
 // low box and the high box of radius r on the cell-sorted pre-step state.
+// Only cells flagged as holding a surface particle (csurf, set by the
+// density pass) are read.
 __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
                         float r, float t) {
-    Box b = scan_box(sc, false, x, r);
-    for (int cy = b.y0; cy <= b.y1; ++cy) {
-        int st, en;
-        row_run(d.cstart, d.cend, sc.cbase_l + cy * sc.dlx, b.x0, b.x1, st, en);
-        for (int j = st; j < en; ++j)
-            if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
-    }
-    Box bh = scan_box(sc, true, x, r);
-    for (int cy = bh.y0; cy <= bh.y1; ++cy) {
-        int st, en;
-        row_run(d.cstart, d.cend, sc.cbase_h + cy * sc.dhx, bh.x0, bh.x1, st,
-                en);
-        for (int j = st; j < en; ++j)
-            if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
+    for (int lv = 0; lv < 2; ++lv) {
+        const bool high = lv == 1;
+        const Box b = scan_box(sc, high, x, r);
+        const int base = high ? sc.cbase_h : sc.cbase_l;
+        const int dimx = high ? sc.dhx : sc.dlx;
+        for (int cy = b.y0; cy <= b.y1; ++cy) {
+            for (int cx = b.x0; cx <= b.x1; ++cx) {
+                const int c = base + cy * dimx + cx;
+                if (!d.csurf[c]) continue;
+                const int en = d.cend[c];
+                for (int j = d.cstart[c]; j < en; ++j)
+                    if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
+            }
+        }
     }
     return false;
 }
@@ -1566,6 +1569,7 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     mark(0);
     const int cur = enqueue_grid(d, sh, st);
     const uint32_t* vals = d.vals[cur];
+    const uint32_t* keys = d.keys[cur];
     for (int it = 0; it < sh.relax_iters; ++it) {
         const int gr = blocks_for(4LL * d.CL, 256, 4096);
         k_relax_density<<<gr, 256, 0, st>>>(d, d.list_h[par], it);
@@ -1573,13 +1577,13 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
         k_relax_commit<<<gr, 256, 0, st>>>(d, d.list_h[par], it);
     }
     mark(1);
-    launch_pair<false, false>(d, vals, sh.tune[kTuneDensityLow], d.CL, st);
+    launch_pair<false, false>(d, vals, keys, sh.tune[kTuneDensityLow], d.CL, st);
     mark(2);
-    launch_pair<false, true>(d, vals, sh.tune[kTuneDensityHigh], 4LL * d.CL, st);
+    launch_pair<false, true>(d, vals, keys, sh.tune[kTuneDensityHigh], 4LL * d.CL, st);
     mark(3);
-    launch_pair<true, false>(d, vals, sh.tune[kTuneForceLow], d.CL, st);
+    launch_pair<true, false>(d, vals, keys, sh.tune[kTuneForceLow], d.CL, st);
     mark(4);
-    launch_pair<true, true>(d, vals, sh.tune[kTuneForceHigh], 4LL * d.CL, st);
+    launch_pair<true, true>(d, vals, keys, sh.tune[kTuneForceHigh], 4LL * d.CL, st);
     mark(5);
     k_reduce<<<sh.red_slots, 256, 0, st>>>(d, vals);
     mark(6);
