@@ -277,6 +277,7 @@ def main() -> int:
                    "force_high": "k_force<true>"}[dom]
     n_now = c1["n_low"] + c1["n_high"]
     cells = c1["cells_low"] + c1["cells_high"]
+    # 8-bit radix digits over the unified cell keys
     passes = max(1, math.ceil(math.ceil(math.log2(max(cells, 2))) / 8))
     gb = grid_bytes(n_now, cells, passes)
     grid_gbs = gb / (ph["grid"] * 1e-3) / 1e9

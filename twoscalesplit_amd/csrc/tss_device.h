@@ -21,7 +21,9 @@
 
 namespace tss {
 
-constexpr int kMaxPasses = 4;     // radix passes (8-bit digits, <= 32-bit keys)
+constexpr int kRadixBits = 8;     // radix digit width
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = (32 + kRadixBits - 1) / kRadixBits;  // 32-bit keys
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort tile
@@ -125,7 +127,7 @@ struct DevState {
     uint32_t* keys[2];        // [5 CL] sort ping-pong
     uint32_t* vals[2];
     uint32_t* hist;           // [256 * max_tiles]
-    uint32_t* digit_tot;      // [kMaxPasses * 256]
+    uint32_t* digit_tot;      // [kRadix] digit totals of the current pass
     int* cstart;
     int* cend;                // [CCL + CCH]
     int* cscene;              // [CCL + CCH] scene owning each cell

@@ -255,7 +255,7 @@ struct tss_batch {
         b += CL * (4 + 4);                         // age scene_of
         b += 2 * (CL + 4 * CL + CL) * 4;           // lists
         b += 4 * U * 4;                            // keys/vals x2
-        b += 256 * tiles * 4 + tss::kMaxPasses * 256 * 4;
+        b += (long long)tss::kRadix * tiles * 4 + tss::kRadix * 4;
         b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 8;
         b += ds.cap_bnd * 12;
         b += U * (8 + 16 + 8 + 1 + 1 + 4 + 4);  // sorted copies + sidx
@@ -310,8 +310,8 @@ struct tss_batch {
             d.vals[i] = dalloc<uint32_t>(U, owned);
         }
         sh.sort_tiles = (int)((U + tss::kSortTile - 1) / tss::kSortTile);
-        d.hist = dalloc<uint32_t>((size_t)256 * sh.sort_tiles, owned);
-        d.digit_tot = dalloc<uint32_t>(tss::kMaxPasses * 256, owned);
+        d.hist = dalloc<uint32_t>((size_t)tss::kRadix * sh.sort_tiles, owned);
+        d.digit_tot = dalloc<uint32_t>(tss::kRadix, owned);
         d.cstart = dalloc<int>(ccl + cch, owned);
         d.cscene = dalloc<int>(ccl + cch, owned);
         d.cend = dalloc<int>(ccl + cch, owned);
@@ -424,7 +424,7 @@ struct tss_batch {
         int bits = 1;
         while ((1LL << bits) < maxkey) ++bits;
         sh.key_bits = bits;
-        sh.passes = (bits + 7) / 8;
+        sh.passes = (bits + tss::kRadixBits - 1) / tss::kRadixBits;
         for (int k = 0; k < tss::kPairKernels; ++k) sh.tune[k] = tune[k];
         sh.relax_iters = 0;
         for (auto& s : scenes)

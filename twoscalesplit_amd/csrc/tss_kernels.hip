@@ -253,17 +253,17 @@ __global__ void k_hash(DevState d, const int* __restrict__ list_l,
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits (replaces thrust::sort_by_key,
+// Stable LSD radix sort, kRadixBits-bit digits (replaces thrust::sort_by_key,
 // R:2828-2834, 2962-2968). Per pass: tile histograms -> per-digit scan over
 // tiles -> stable scatter ranked with wavefront ballots.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(
     const uint32_t* __restrict__ keys, const DevCounts* __restrict__ cnt,
     int shift, uint32_t* __restrict__ hist, int max_tiles) {
-    __shared__ uint32_t h[256];
+    __shared__ uint32_t h[kRadix];
     const int n = cnt->n_l + cnt->n_h;
     const int tile = blockIdx.x;
-    h[threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < kRadix; i += kSortThreads) h[i] = 0;
     __syncthreads();
     const int base = tile * kSortTile;
     const int lane = threadIdx.x & 63;
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(
         for (int i = 0; i < kSortItems; ++i) {
             const int e = base + i * kSortThreads + threadIdx.x;
             const bool valid = e < n;
-            const uint32_t dg = valid ? (keys[e] >> shift) & 255u : 256u;
+            const uint32_t dg = valid ? (keys[e] >> shift) & (kRadix - 1u) : (uint32_t)kRadix;
             const uint32_t prev = (uint32_t)__shfl_up((int)dg, 1, 64);
             const bool start = valid && (lane == 0 || prev != dg);
             const uint64_t starts = __ballot(start);
@@ -287,7 +287,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(
         }
     }
     __syncthreads();
-    hist[threadIdx.x * max_tiles + tile] = h[threadIdx.x];
+    for (int i = threadIdx.x; i < kRadix; i += kSortThreads)
+        hist[(size_t)i * max_tiles + tile] = h[i];
 }
 
 // One block per digit: exclusive scan over the tiles of that digit.
@@ -326,6 +327,12 @@ __global__ __launch_bounds__(256) void k_sort_scan(
     if (threadIdx.x == 255) digit_tot[blockIdx.x] = part[255];
 }
 
+// Stable scatter of one tile. Per item (256 consecutive keys): each wave
+// ranks its lanes among equal digits with kRadixBits ballots; the first
+// lane of each digit group (its leader) publishes the group's size, reads
+// the sizes of the other waves' groups of that digit and the digit's
+// running count in the tile, and passes the group's offset to its lanes.
+// The work per item does not depend on the number of digits.
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -333,10 +340,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ digit_tot,
     int max_tiles) {
     constexpr int kWaves = kSortThreads / 64;
-    __shared__ uint32_t base[256];
-    __shared__ uint32_t running[256];
-    __shared__ uint32_t wcnt[kWaves][256];
-    __shared__ uint32_t woff[kWaves][256];
+    constexpr int kPer = kRadix / kSortThreads;  // digits per thread
+    __shared__ uint32_t base[kRadix];
+    __shared__ uint32_t running[kRadix];
+    __shared__ uint32_t wcnt[kWaves][kRadix];
+    __shared__ uint32_t part[kSortThreads];
     const int n = cnt->n_l + cnt->n_h;
     const int tile = blockIdx.x;
     const int tbase = tile * kSortTile;
@@ -344,56 +352,77 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
-    // digit bases: exclusive scan of digit totals + this tile's prefix
+    // digit bases: exclusive scan of the digit totals + this tile's prefix
     {
-        uint32_t v = digit_tot[t];
-        base[t] = v;
+        uint32_t loc[kPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            loc[q] = digit_tot[t * kPer + q];
+            sum += loc[q];
+        }
+        part[t] = sum;
         __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            uint32_t a = t >= off ? base[t - off] : 0u;
+        for (int off = 1; off < kSortThreads; off <<= 1) {
+            uint32_t a = t >= off ? part[t - off] : 0u;
             __syncthreads();
-            base[t] += a;
+            part[t] += a;
             __syncthreads();
         }
-        uint32_t excl = base[t] - v;
-        __syncthreads();
-        base[t] = excl + hist[t * max_tiles + tile];
-        running[t] = 0;
-        for (int q = 0; q < kWaves; ++q) wcnt[q][t] = 0;
+        uint32_t run = part[t] - sum;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int dgt = t * kPer + q;
+            base[dgt] = run + hist[(size_t)dgt * max_tiles + tile];
+            run += loc[q];
+            running[dgt] = 0;
+#pragma unroll
+            for (int v = 0; v < kWaves; ++v) wcnt[v][dgt] = 0;
+        }
     }
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int it = 0; it < kSortItems; ++it) {
         const int e = tbase + it * kSortThreads + t;
         const bool valid = e < n;
-        uint32_t key = valid ? kin[e] : 0u;
-        uint32_t val = valid ? vin[e] : 0u;
-        uint32_t dg = (key >> shift) & 255u;
+        const uint32_t key = valid ? kin[e] : 0u;
+        const uint32_t val = valid ? vin[e] : 0u;
+        const uint32_t dg = (key >> shift) & (kRadix - 1u);
         uint64_t m = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            uint64_t bb = __ballot((dg >> b) & 1u);
+        for (int b = 0; b < kRadixBits; ++b) {
+            const uint64_t bb = __ballot((dg >> b) & 1u);
             m &= ((dg >> b) & 1u) ? bb : ~bb;
         }
         const uint32_t rk = (uint32_t)__popcll(m & lt);
-        if (valid && rk == 0) wcnt[w][dg] = (uint32_t)__popcll(m);
+        const bool leader = valid && rk == 0;
+        if (leader) wcnt[w][dg] = (uint32_t)__popcll(m);
         __syncthreads();
-        {
-            uint32_t run = running[t];
-            for (int q = 0; q < kWaves; ++q) {
-                woff[q][t] = run;
-                run += wcnt[q][t];
-                wcnt[q][t] = 0;
+        uint32_t off = 0, tot = 0;
+        bool last = true;
+        if (leader) {
+#pragma unroll
+            for (int v = 0; v < kWaves; ++v) {
+                const uint32_t c = wcnt[v][dg];
+                off += v < w ? c : 0u;
+                tot += c;
+                last = last && (v <= w || c == 0u);
             }
-            running[t] = run;
+            off += running[dg];
         }
         __syncthreads();
+        if (leader) {
+            if (last) running[dg] += tot;  // the digit's last wave this item
+            wcnt[w][dg] = 0;
+        }
+        // the group's offset, from its leader (the lowest lane of m)
+        const int src = valid ? __ffsll((long long)m) - 1 : lane;
+        off = (uint32_t)__shfl((int)off, src, 64);
         if (valid) {
-            uint32_t dst = base[dg] + woff[w][dg] + rk;
+            const uint32_t dst = base[dg] + off + rk;
             kout[dst] = key;
             vout[dst] = val;
         }
-        __syncthreads();
     }
 }
 
@@ -1539,11 +1568,11 @@ static int enqueue_grid(const DevState& d, const StepShape& sh,
                                          d.keys[0], d.vals[0]);
     int cur = 0;
     for (int pass = 0; pass < sh.passes; ++pass) {
-        const int shift = 8 * pass;
+        const int shift = kRadixBits * pass;
         k_sort_hist<<<sh.sort_tiles, kSortThreads, 0, st>>>(
             d.keys[cur], d.counts, shift, d.hist, sh.sort_tiles);
-        k_sort_scan<<<256, 256, 0, st>>>(d.hist, d.counts, sh.sort_tiles,
-                                          d.digit_tot);
+        k_sort_scan<<<kRadix, 256, 0, st>>>(d.hist, d.counts, sh.sort_tiles,
+                                             d.digit_tot);
         k_sort_scatter<<<sh.sort_tiles, kSortThreads, 0, st>>>(
             d.keys[cur], d.vals[cur], d.keys[cur ^ 1], d.vals[cur ^ 1],
             d.counts, shift, d.hist, d.digit_tot, sh.sort_tiles);
