@@ -189,9 +189,10 @@ def main() -> int:
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="scenes of the batched dam-break mix (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=300)
-    ap.add_argument("--serve-requests", type=int, default=16,
+    ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the served run (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=16.0)
+    ap.add_argument("--serve-rate", type=float, default=64.0,
+                    help="Poisson arrival rate of the served run (req/s)")
     ap.add_argument("--copy-probe-mb", type=int, default=512,
                     help="plain-copy bandwidth probe size (0: skip); also "
                          "the reference dispatch for HBM counter calibration")
