@@ -21,6 +21,7 @@
 
 namespace tss {
 
+constexpr int kClearChunk = 4096;  // cells per block of the cell-table clear
 constexpr int kRadixBits = 8;     // radix digit width
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = (32 + kRadixBits - 1) / kRadixBits;  // 32-bit keys
@@ -131,6 +132,10 @@ struct DevState {
     int* cstart;
     int* cend;                // [CCL + CCH]
     int* cscene;              // [CCL + CCH] scene owning each cell
+    // [cells / kClearChunk + 2 S + 1] cell ranges [x, y) of the resident
+    // scenes, at most kClearChunk cells each: one block of the step's
+    // cell-table clear per entry
+    int2* clear_blk;
     int* bstart;
     int* bend;                // [CCL]
     float2* bpos;             // [NB] boundary positions in cell order
@@ -184,6 +189,7 @@ struct StepShape {
     int red_slots;            // chunk slots in use (grid of the reduction)
     int list_parity;          // which list buffer holds the current lists
     int relax_iters;          // max relaxation iterations over the scenes
+    int clear_blocks;         // blocks of the cell-table clear (clear_blk)
 };
 
 // Timing points of one step (events recorded before / after each phase):

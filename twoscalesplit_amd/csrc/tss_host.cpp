@@ -266,6 +266,7 @@ struct tss_batch {
         b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
         b += 3 * (ds.cells_low + ds.cells_high);  // cell state and surface flags
         b += 4 * (ds.cells_low + ds.cells_high);  // cell -> scene
+        b += 8 * ((ds.cells_low + ds.cells_high) / tss::kClearChunk + 2LL * ds.max_scenes + 1);
         b += 16 * 4 * CL;                         // relaxation scratch (when used)
         return b;
     }
@@ -340,6 +341,8 @@ struct tss_batch {
         red_cap = (int)((U + tss::kRedChunk - 1) / tss::kRedChunk) + max_scenes + 1;
         d.red = dalloc<double>((size_t)red_cap * tss::kRedFields, owned);
         d.red_scene = dalloc<int>(red_cap, owned);
+        clear_cap = (int)((ccl + cch) / tss::kClearChunk) + 2 * max_scenes + 1;
+        d.clear_blk = dalloc<int2>(clear_cap, owned);
         d.red_base = dalloc<int>(max_scenes, owned);
         d.red_n = dalloc<int>(max_scenes, owned);
         d.cons = dalloc<double>((size_t)max_scenes * 10, owned);
@@ -415,9 +418,33 @@ struct tss_batch {
     }
 
     int red_cap = 0;
+    int clear_cap = 0;
+
+    // The cell tables are cleared at the start of every step only over the
+    // resident scenes' cell ranges (a pool sized for many scenes does not
+    // pay for its empty capacity): one block per kClearChunk cells.
+    void upload_clear_table() {
+        std::vector<int2> blk;
+        auto add = [&](long long a, long long n) {
+            for (long long x = a; x < a + n; x += tss::kClearChunk)
+                blk.push_back(make_int2((int)x, (int)std::min(a + n, x + tss::kClearChunk)));
+        };
+        for (const SceneHost& s : scenes)
+            if (s.alive) {
+                add(s.cbl, s.ncl);
+                add(s.cbh, s.nch);
+            }
+        if (blk.empty()) blk.push_back(make_int2(0, 0));
+        require((int)blk.size() <= clear_cap, "cell-clear table exhausted");
+        sh.clear_blocks = (int)blk.size();
+        HIP_CHECK(hipMemcpyAsync(d.clear_blk, blk.data(), sizeof(int2) * blk.size(),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
 
     void update_shape() {
         upload_reduction_tables();
+        upload_clear_table();
         long long maxkey = 1;
         for (auto& s : scenes)
             if (s.alive) maxkey = std::max<long long>(maxkey, (long long)s.cbh + s.nch);

@@ -197,10 +197,11 @@ __global__ void k_fill_int(int* __restrict__ p, int v, long long n) {
         p[i] = v;
 }
 
-// Start of a step: clear cell tables and per-scene step accumulators.
-__global__ void k_step_begin(DevState d, long long ncells) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-         i < ncells; i += (long long)gridDim.x * blockDim.x) {
+// Start of a step: clear the cell tables of the resident scenes (block b
+// clears the cells of range clear_blk[b]) and per-scene step accumulators.
+__global__ void k_step_begin(DevState d) {
+    const int2 r = d.clear_blk[blockIdx.x];
+    for (int i = r.x + (int)threadIdx.x; i < r.y; i += (int)blockDim.x) {
         d.cstart[i] = -1;
         d.cend[i] = -1;
         d.cflag[2 * i] = 0;
@@ -1562,8 +1563,7 @@ void launch_fill_int(int* p, int value, long long n, void* stream) {
 static int enqueue_grid(const DevState& d, const StepShape& sh,
                         hipStream_t st) {
     const int par = sh.list_parity;
-    const long long ncells = (long long)d.CCL + d.CCH;
-    k_step_begin<<<blocks_for(ncells, 256, 4096), 256, 0, st>>>(d, ncells);
+    k_step_begin<<<sh.clear_blocks, 256, 0, st>>>(d);
     k_hash<<<sh.grid_elem, 256, 0, st>>>(d, d.list_l[par], d.list_h[par],
                                          d.keys[0], d.vals[0]);
     int cur = 0;
