@@ -264,6 +264,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(
     __shared__ uint32_t h[kRadix];
     const int n = cnt->n_l + cnt->n_h;
     const int tile = blockIdx.x;
+    // tiles past the live keys (the grid is sized by capacity) are never
+    // read by the scan: the whole block exits
+    if (tile * kSortTile >= n) return;
     for (int i = threadIdx.x; i < kRadix; i += kSortThreads) h[i] = 0;
     __syncthreads();
     const int base = tile * kSortTile;
@@ -932,6 +935,15 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_reduce(
     const int nl = d.counts->n_l, nh = d.counts->n_h, ng = d.counts->n_g;
     int seg, t;
     tile_seg(sh, blockIdx.x, seg, t);
+    // a tile past its segment's live entries counts nothing (the grid is
+    // sized by capacity): one write, and the whole block exits
+    if (t * kScanTile >= (seg == 0 ? nl : seg == 1 ? nh : ng)) {
+        if (threadIdx.x == 0) {
+            d.tile_sums[2 * blockIdx.x] = 0;
+            d.tile_sums[2 * blockIdx.x + 1] = 0;
+        }
+        return;
+    }
     int a = 0, b = 0;  // wave totals (wave-uniform)
     const int i0 = t * kScanTile + (int)threadIdx.x;
     for (int q = 0; q < kScanItems; ++q) {
@@ -1023,6 +1035,8 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
     const int nl = d.counts->n_l, nh = d.counts->n_h, ng = d.counts->n_g;
     int seg, t;
     tile_seg(sh, blockIdx.x, seg, t);
+    // tiles past their segment's live entries write nothing: whole block
+    if (t * kScanTile >= (seg == 0 ? nl : seg == 1 ? nh : ng)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     // output positions of the tile's first entry
