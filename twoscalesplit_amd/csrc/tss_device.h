@@ -181,9 +181,14 @@ enum { kTuneDensityLow = 0, kTuneDensityHigh, kTuneForceLow, kTuneForceHigh,
 struct StepShape {
     int passes;               // radix passes
     int key_bits;
+    // Grids follow the resident scenes' capacity (cap_alive low ids, 4x
+    // as many high): a pool sized for many scenes launches for those it
+    // holds. Every kernel loops over or tests the device-side counts.
+    long long cap_alive;      // low-id capacity of the resident scenes
     int grid_elem;            // blocks of element-wise kernels
-    int sort_tiles;           // max sort tiles (capacity-based)
-    int scan_tiles_l, scan_tiles_h, scan_tiles_g;  // capacity-based tiles
+    int sort_tiles;           // sort tiles allocated (row stride of hist)
+    int sort_grid;            // sort tiles launched (<= sort_tiles)
+    int scan_tiles_l, scan_tiles_h, scan_tiles_g;  // compaction tiles launched
     PairTune tune[kPairKernels];
     int S;
     int red_slots;            // chunk slots in use (grid of the reduction)

@@ -331,10 +331,8 @@ struct tss_batch {
         d.nmask = dalloc<uint32_t>((size_t)U * tss::kMaskWords, owned);
         d.cand = dalloc<uint8_t>(CL, owned);
         d.merged = dalloc<uint8_t>(CL, owned);
-        sh.scan_tiles_l = (int)((CL + tss::kScanTile - 1) / tss::kScanTile);
-        sh.scan_tiles_h = (int)((4 * CL + tss::kScanTile - 1) / tss::kScanTile);
-        sh.scan_tiles_g = sh.scan_tiles_l;
-        const int nst = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
+        const int nst = (int)((CL + tss::kScanTile - 1) / tss::kScanTile) * 2 +
+                        (int)((4 * CL + tss::kScanTile - 1) / tss::kScanTile);
         d.tile_sums = dalloc<int>(2 * (size_t)nst, owned);
         d.tile_base = dalloc<int>(2 * (size_t)nst, owned);
         d.seg_base = dalloc<int>(8, owned);
@@ -377,12 +375,8 @@ struct tss_batch {
         a_cl.reset(ccl);
         a_ch.reset(cch);
         scenes.assign(max_scenes, SceneHost{});
-        // element-wise kernels loop over the device-side counts, so their
-        // grid is capped: a batch sized well above its live population
-        // (serving) does not launch idle blocks by the hundred thousand
-        sh.grid_elem = (int)std::min<long long>(kElemGridCap, std::max<long long>(1, (U + 255) / 256));
-        // likewise the neighbour kernels default to a persistent grid when
-        // a capacity-sized grid would exceed kPairGridFull blocks
+        // the neighbour kernels default to a persistent grid when a
+        // capacity-sized grid would exceed kPairGridFull blocks
         for (int k = 0; k < tss::kPairKernels; ++k) {
             const long long n = (k == tss::kTuneDensityHigh || k == tss::kTuneForceHigh) ? 4 * CL : CL;
             if ((n + tune[k].block - 1) / tune[k].block > kPairGridFull) tune[k].grid = kPairGridCap;
@@ -445,6 +439,18 @@ struct tss_batch {
     void update_shape() {
         upload_reduction_tables();
         upload_clear_table();
+        // launch sizes from the resident scenes' capacity
+        long long cap = 0;
+        for (const SceneHost& s : scenes)
+            if (s.alive) cap += s.lcap;
+        cap = std::max<long long>(cap, 1);
+        const long long U = 5 * cap;
+        sh.cap_alive = cap;
+        sh.grid_elem = (int)std::min<long long>(kElemGridCap, (U + 255) / 256);
+        sh.sort_grid = (int)std::min<long long>(sh.sort_tiles, (U + tss::kSortTile - 1) / tss::kSortTile);
+        sh.scan_tiles_l = (int)((cap + tss::kScanTile - 1) / tss::kScanTile);
+        sh.scan_tiles_h = (int)((4 * cap + tss::kScanTile - 1) / tss::kScanTile);
+        sh.scan_tiles_g = sh.scan_tiles_l;
         long long maxkey = 1;
         for (auto& s : scenes)
             if (s.alive) maxkey = std::max<long long>(maxkey, (long long)s.cbh + s.nch);

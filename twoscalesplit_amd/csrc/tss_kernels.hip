@@ -264,8 +264,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(
     __shared__ uint32_t h[kRadix];
     const int n = cnt->n_l + cnt->n_h;
     const int tile = blockIdx.x;
-    // tiles past the live keys (the grid is sized by capacity) are never
-    // read by the scan: the whole block exits
+    // tiles past the live keys (the grid is sized by the resident scenes'
+    // capacity) are never read by the scan: the whole block exits
     if (tile * kSortTile >= n) return;
     for (int i = threadIdx.x; i < kRadix; i += kSortThreads) h[i] = 0;
     __syncthreads();
@@ -1583,11 +1583,11 @@ static int enqueue_grid(const DevState& d, const StepShape& sh,
     int cur = 0;
     for (int pass = 0; pass < sh.passes; ++pass) {
         const int shift = kRadixBits * pass;
-        k_sort_hist<<<sh.sort_tiles, kSortThreads, 0, st>>>(
+        k_sort_hist<<<sh.sort_grid, kSortThreads, 0, st>>>(
             d.keys[cur], d.counts, shift, d.hist, sh.sort_tiles);
         k_sort_scan<<<kRadix, 256, 0, st>>>(d.hist, d.counts, sh.sort_tiles,
                                              d.digit_tot);
-        k_sort_scatter<<<sh.sort_tiles, kSortThreads, 0, st>>>(
+        k_sort_scatter<<<sh.sort_grid, kSortThreads, 0, st>>>(
             d.keys[cur], d.vals[cur], d.keys[cur ^ 1], d.vals[cur ^ 1],
             d.counts, shift, d.hist, d.digit_tot, sh.sort_tiles);
         cur ^= 1;
@@ -1614,24 +1614,24 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     const uint32_t* vals = d.vals[cur];
     const uint32_t* keys = d.keys[cur];
     for (int it = 0; it < sh.relax_iters; ++it) {
-        const int gr = blocks_for(4LL * d.CL, 256, 4096);
+        const int gr = blocks_for(4 * sh.cap_alive, 256, 4096);
         k_relax_density<<<gr, 256, 0, st>>>(d, d.list_h[par], it);
         k_relax_force<<<gr, 256, 0, st>>>(d, vals, d.list_h[par], it);
         k_relax_commit<<<gr, 256, 0, st>>>(d, d.list_h[par], it);
     }
     mark(1);
-    launch_pair<false, false>(d, vals, keys, sh.tune[kTuneDensityLow], d.CL, st);
+    launch_pair<false, false>(d, vals, keys, sh.tune[kTuneDensityLow], sh.cap_alive, st);
     mark(2);
-    launch_pair<false, true>(d, vals, keys, sh.tune[kTuneDensityHigh], 4LL * d.CL, st);
+    launch_pair<false, true>(d, vals, keys, sh.tune[kTuneDensityHigh], 4 * sh.cap_alive, st);
     mark(3);
-    launch_pair<true, false>(d, vals, keys, sh.tune[kTuneForceLow], d.CL, st);
+    launch_pair<true, false>(d, vals, keys, sh.tune[kTuneForceLow], sh.cap_alive, st);
     mark(4);
-    launch_pair<true, true>(d, vals, keys, sh.tune[kTuneForceHigh], 4LL * d.CL, st);
+    launch_pair<true, true>(d, vals, keys, sh.tune[kTuneForceHigh], 4 * sh.cap_alive, st);
     mark(5);
     k_reduce<<<sh.red_slots, 256, 0, st>>>(d, vals);
     mark(6);
     k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
-    k_groups<<<blocks_for(d.CL, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
+    k_groups<<<blocks_for(sh.cap_alive, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
     k_cap<<<sh.S, 256, 0, st>>>(d);
     const int ntiles = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
     k_compact_reduce<<<ntiles, kScanThreads, 0, st>>>(d, sh, vals,
