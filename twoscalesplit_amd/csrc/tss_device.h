@@ -36,6 +36,9 @@ constexpr int kScanTile = kScanThreads * kScanItems;  // entries per compaction 
 // own capacity, so the partition -- and every rounding -- is the same alone
 // and in a batch).
 constexpr int kRedChunk = 4096;
+// Spare entries past the capacity of the cell-sorted position and state
+// arrays: the candidate fill reads up to 31 entries beyond a run's end.
+constexpr int kSortedPad = 32;
 // mass, px, py, ke, pe, max|v|^2, sum|px|, sum|py|, sum|pe|
 constexpr int kRedFields = 9;
 
@@ -140,11 +143,11 @@ struct DevState {
     int* bend;                // [CCL]
     float2* bpos;             // [NB] boundary positions in cell order
     int* bids;                // [NB] boundary ids (scene-local) in cell order
-    float2* spos;
+    float2* spos;             // [5 CL + kSortedPad]
     float4* sxv;              // [5 CL] cell-sorted (x, y, vx, vy)
     float4* snew;             // [5 CL] cell-sorted state after the step
     float2* srp;              // [5 CL] cell-sorted (rho, p / rho^2)
-    uint8_t* sstate;
+    uint8_t* sstate;          // [5 CL + kSortedPad]
     uint8_t* ssurf;           // surface flag (sorted)
     int* sscene;
     int* sidx;                // [5 CL] sorted index of unified id

@@ -259,6 +259,7 @@ struct tss_batch {
         b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 8;
         b += ds.cap_bnd * 12;
         b += U * (8 + 16 + 8 + 1 + 1 + 4 + 4);  // sorted copies + sidx
+        b += tss::kSortedPad * (8 + 1);
         b += U * 16;                               // sorted post-step state
         b += U * 4 * tss::kMaskWords;              // candidate masks
         b += CL * 2;
@@ -320,11 +321,11 @@ struct tss_batch {
         d.bend = dalloc<int>(ccl, owned);
         d.bpos = dalloc<float2>(cap_b, owned);
         d.bids = dalloc<int>(cap_b, owned);
-        d.spos = dalloc<float2>(U, owned);
+        d.spos = dalloc<float2>(U + tss::kSortedPad, owned);
         d.sxv = dalloc<float4>(U, owned);
         d.snew = dalloc<float4>(U, owned);
         d.srp = dalloc<float2>(U, owned);
-        d.sstate = dalloc<uint8_t>(U, owned);
+        d.sstate = dalloc<uint8_t>(U + tss::kSortedPad, owned);
         d.ssurf = dalloc<uint8_t>(U, owned);
         d.sscene = dalloc<int>(U, owned);
         d.sidx = dalloc<int>(U, owned);

@@ -59,6 +59,16 @@ __device__ __forceinline__ T gather(const T* __restrict__ base, uint32_t i) {
                                        (size_t)(i * (uint32_t)sizeof(T)));
 }
 
+// Element j + i through the 32-bit byte offset of j, i a small constant: the
+// loads of j + 0, 1, ... share one address and differ in their immediate
+// offsets.
+template <class T>
+__device__ __forceinline__ T gather_at(const T* __restrict__ base, uint32_t j, int i) {
+    const T* p = reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                            (size_t)(j * (uint32_t)sizeof(T)));
+    return p[i];
+}
+
 // 0 < r2 <= t as one unsigned comparison of the bit patterns (r2 is a sum
 // of squares, so never negative; NaN compares above every finite t).
 __device__ __forceinline__ bool in_open_ball(float r2, float t) {
@@ -178,10 +188,10 @@ __device__ __forceinline__ float fast_rcp(float x) {
     return __builtin_amdgcn_rcpf(x);
 }
 
-// |x_ij| from r2 >= 0 without the 0 * inf of r2 * rsq(r2) at r2 = 0 (the
-// self pair of the density sums).
+// |x_ij| from r2 >= 0: the hardware square root (one instruction, exact
+// at r2 = 0, the self pair of the density sums).
 __device__ __forceinline__ float dist_from2(float r2) {
-    return r2 > 0.0f ? r2 * fast_rsq(r2) : 0.0f;
+    return __builtin_amdgcn_sqrtf(r2);
 }
 
 __device__ __forceinline__ int low_of(const DevState& d, int u) {
