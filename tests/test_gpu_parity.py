@@ -136,6 +136,17 @@ def test_promote_merge_dissolve_parity():
     assert ev[0] > 0 and ev[1] > 0, ev
 
 
+def test_wide_high_radius_parity():
+    # H = 0.6 h: a high target's own pairs reach past 2r/h = 1, so the high
+    # passes take the general blended kernel (H = h/2 scenes take W_h's
+    # inner branch)
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND, h_ratio=0.6)
+    rep = []
+    _run_checks(sc, [0, 0, 0, 5], rep)
+    print(json.dumps(rep))
+    assert rep[0]["events"][0] > 0, "the first step must split the surface band"
+
+
 def test_cfl_dt_parity():
     sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND, dt_cfl=True)
     rep = []

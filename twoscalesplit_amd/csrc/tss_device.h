@@ -78,6 +78,7 @@ struct SceneDev {
     int alive;
     int relax_iters;          // relaxation iterations of new children
     float relax_frac;         // relaxation step = relax_frac * dt
+    int hh_inner;             // 1: 2H <= h, so r <= H gives q_h = 2r/h <= 1
 };
 
 struct SceneCount {

@@ -561,6 +561,7 @@ struct tss_batch {
         s.alive = 1;
         s.relax_iters = c.relax_iters;
         s.relax_frac = c.relax_dt_frac;
+        s.hh_inner = 2.0 * (double)s.H <= (double)s.h ? 1 : 0;
         return s;
     }
 

@@ -132,6 +132,33 @@ __device__ __forceinline__ void m4_blend(const SceneDev& sc, float r, float& w,
     g = fmaf(sc.gc_h2, dh, sc.gc_H2 * dH);
 }
 
+// m4_blend for pairs with q_h = 2r/h <= 1 (a high target's pairs at r <= H
+// when 2H <= h): W_h takes its inner branch, (2-q)^3 - 4(1-q)^3 =
+// 4 - 6q^2 + 3q^3, with a^2 - 4b^2 = q(4 - 3q); W_H as in m4_blend. Rounding
+// can put q an ulp or two above 1, where both branches agree to first
+// order.
+__device__ __forceinline__ void m4_blend_inner(const SceneDev& sc, float r, float& w,
+                                               float& g) {
+    float qh = r * sc.q_h, qH = r * sc.q_H;
+    float aH = fmaxf(2.0f - qH, 0.0f), bH = fmaxf(1.0f - qH, 0.0f);
+    float aH2 = aH * aH, bH2 = bH * bH;
+    float fh = fmaf(qh * qh, fmaf(3.0f, qh, -6.0f), 4.0f);
+    float fH = fmaf(-4.0f * bH, bH2, aH2 * aH);
+    float dh = qh * fmaf(-3.0f, qh, 4.0f);
+    float dH = fmaf(-4.0f, bH2, aH2);
+    w = fmaf(sc.sig_h2, fh, sc.sig_H2 * fH);
+    g = fmaf(sc.gc_h2, dh, sc.gc_H2 * dH);
+}
+
+// m4_blend, or m4_blend_inner when INNER (chosen per scene, hh_inner).
+template <bool INNER>
+__device__ __forceinline__ void m4_pick(const SceneDev& sc, float r, float& w, float& g) {
+    if (INNER)
+        m4_blend_inner(sc, r, w, g);
+    else
+        m4_blend(sc, r, w, g);
+}
+
 // Wall weight Gamma(r)/(mass factor) (R:66-114), qs = 2/ell.
 __device__ __forceinline__ float wall_gamma(const SceneDev& sc, float r2,
                                             float r, float qs) {

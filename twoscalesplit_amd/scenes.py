@@ -79,9 +79,9 @@ def generate(seed: int, scene: int, nx: int, ny: int, dx: float, nxd: int,
 
 def _config(width: float, height: float, dx: float, two_scale: bool,
             alpha: float = 0.05, dt: float = 1e-4, cfl: bool = False,
-            block_h: float = BLOCK_H) -> N.TssConfig:
+            block_h: float = BLOCK_H, h_ratio: float = 0.5) -> N.TssConfig:
     h = np.float32(4.0 * dx)
-    H = np.float32(h / 2) if two_scale else h
+    H = np.float32(h * np.float32(h_ratio)) if two_scale else h
     c0 = 10.0 * math.sqrt(2.0 * G * block_h)
     B = RHO0 * c0 * c0 / 7.0
     dt0 = dt
@@ -204,11 +204,11 @@ def c5_request(seed: int, idx: int, lo: int = 4000, hi: int = 256000,
 def small_two_scale(seed: int = SEED, nx: int = 16, ny: int = 24,
                     mode: int = N.MODE_BAND, nxd: int = 48, nyd: int = 40,
                     dx: float = 1.0 / 160.0, split_x: float = 0.5,
-                    dt_cfl: bool = False) -> Scene:
-    """Small scene for parity tests (oracle runs in seconds)."""
+                    dt_cfl: bool = False, h_ratio: float = 0.5) -> Scene:
+    """Small scene for parity tests (oracle runs in seconds); h_ratio = H/h."""
     low, bnd = generate(seed, 7, nx, ny, dx, nxd, nyd)
     cfg = _config(nxd * dx, nyd * dx, dx, two_scale=mode != N.MODE_PLAIN,
-                  cfl=dt_cfl)
+                  cfl=dt_cfl, h_ratio=h_ratio)
     cfg.mode = mode
     cfg.split_x = split_x
     cfg.cap_high = 4 * nx * ny if mode != N.MODE_PLAIN else 0
