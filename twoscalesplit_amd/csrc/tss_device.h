@@ -142,6 +142,10 @@ struct DevState {
     int2* clear_blk;
     int* bstart;
     int* bend;                // [CCL]
+    // [CCL] 1: a boundary particle in the low cells x..x+3, y..y+3 of this
+    // cell (x, y); a box of at most 4x4 low cells starting at a cell
+    // without the flag has no wall particle
+    uint8_t* bnear;
     float2* bpos;             // [NB] boundary positions in cell order
     int* bids;                // [NB] boundary ids (scene-local) in cell order
     float2* spos;             // [5 CL + kSortedPad]

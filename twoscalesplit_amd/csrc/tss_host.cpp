@@ -256,7 +256,7 @@ struct tss_batch {
         b += 2 * (CL + 4 * CL + CL) * 4;           // lists
         b += 4 * U * 4;                            // keys/vals x2
         b += (long long)tss::kRadix * tiles * 4 + tss::kRadix * 4;
-        b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 8;
+        b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 9;
         b += ds.cap_bnd * 12;
         b += U * (8 + 16 + 8 + 1 + 1 + 4 + 4);  // sorted copies + sidx
         b += tss::kSortedPad * (8 + 1);
@@ -319,6 +319,7 @@ struct tss_batch {
         d.cend = dalloc<int>(ccl + cch, owned);
         d.bstart = dalloc<int>(ccl, owned);
         d.bend = dalloc<int>(ccl, owned);
+        d.bnear = dalloc<uint8_t>(ccl, owned);
         d.bpos = dalloc<float2>(cap_b, owned);
         d.bids = dalloc<int>(cap_b, owned);
         d.spos = dalloc<float2>(U + tss::kSortedPad, owned);
@@ -709,6 +710,26 @@ struct tss_batch {
         }
         HIP_CHECK(hipMemcpyAsync(d.bstart + cl, bst.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipMemcpyAsync(d.bend + cl, ben.data(), sizeof(int) * ncl, hipMemcpyHostToDevice, stream));
+        // wall-near flags: any boundary particle in the 4x4 cells from (x, y)
+        // (an OR over x..x+3, then over y..y+3)
+        std::vector<uint8_t> near(ncl, 0);
+        {
+            const int nx = c.domain_dims[0], ny = c.domain_dims[1];
+            std::vector<uint8_t> rowor(ncl, 0);
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    uint8_t f = 0;
+                    for (int i = x; i < std::min(x + 4, nx); ++i) f |= bst[(size_t)y * nx + i] >= 0;
+                    rowor[(size_t)y * nx + x] = f;
+                }
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    uint8_t f = 0;
+                    for (int j = y; j < std::min(y + 4, ny); ++j) f |= rowor[(size_t)j * nx + x];
+                    near[(size_t)y * nx + x] = f;
+                }
+        }
+        HIP_CHECK(hipMemcpyAsync(d.bnear + cl, near.data(), ncl, hipMemcpyHostToDevice, stream));
         // initial low list: ids in lattice order (the reference's iota list,
         // R/WCSPHSolver.cu:2358-2371), appended to the current list
         const int par = sh.list_parity;
