@@ -304,18 +304,26 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(
     // tiles past the live keys (the grid is sized by the resident scenes'
     // capacity) are never read by the scan: the whole block exits
     if (tile * kSortTile >= n) return;
+    const int base = tile * kSortTile;
+    // all of the thread's keys first: their loads are in flight together
+    uint32_t kk[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int e = base + i * kSortThreads + threadIdx.x;
+        kk[i] = e < n ? keys[e] : 0u;
+    }
     for (int i = threadIdx.x; i < kRadix; i += kSortThreads) h[i] = 0;
     __syncthreads();
-    const int base = tile * kSortTile;
     const int lane = threadIdx.x & 63;
-    if (base < n) {
+    {
         // keys arrive nearly sorted (lists are kept in cell order), so
         // equal digits come in runs along the lanes: one LDS atomic per
         // run, from the run's first lane (valid lanes are a prefix)
+#pragma unroll
         for (int i = 0; i < kSortItems; ++i) {
             const int e = base + i * kSortThreads + threadIdx.x;
             const bool valid = e < n;
-            const uint32_t dg = valid ? (keys[e] >> shift) & (kRadix - 1u) : (uint32_t)kRadix;
+            const uint32_t dg = valid ? (kk[i] >> shift) & (kRadix - 1u) : (uint32_t)kRadix;
             const uint32_t prev = (uint32_t)__shfl_up((int)dg, 1, 64);
             const bool start = valid && (lane == 0 || prev != dg);
             const uint64_t starts = __ballot(start);
@@ -393,6 +401,15 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
+    // all of the thread's keys and values first: their loads are in flight
+    // together (and behind the digit-base scan below)
+    uint32_t kk[kSortItems], vv[kSortItems];
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int e = tbase + it * kSortThreads + t;
+        kk[it] = e < n ? kin[e] : 0u;
+        vv[it] = e < n ? vin[e] : 0u;
+    }
     // digit bases: exclusive scan of the digit totals + this tile's prefix
     {
         uint32_t loc[kPer];
@@ -423,11 +440,12 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     }
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
         const int e = tbase + it * kSortThreads + t;
         const bool valid = e < n;
-        const uint32_t key = valid ? kin[e] : 0u;
-        const uint32_t val = valid ? vin[e] : 0u;
+        const uint32_t key = kk[it];
+        const uint32_t val = vv[it];
         const uint32_t dg = (key >> shift) & (kRadix - 1u);
         uint64_t m = __ballot(valid);
 #pragma unroll
