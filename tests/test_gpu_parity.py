@@ -147,6 +147,19 @@ def test_wide_high_radius_parity():
     assert rep[0]["events"][0] > 0, "the first step must split the surface band"
 
 
+@pytest.mark.parametrize("h_ratio", [0.5, 0.6])
+def test_tension_parity(h_ratio):
+    # sigma != 0: the force passes add -sigma x_ij W_bar per pair (the
+    # tension variants of the mask walk and of the scans), on both kernels
+    # of the high level (inner branch at H = h/2, general at H = 0.6 h)
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND, h_ratio=h_ratio)
+    sc.cfg.tension_coefficient = 0.3
+    rep = []
+    _run_checks(sc, [0, 0, 0, 5], rep)
+    print(json.dumps(rep))
+    assert rep[0]["events"][0] > 0, "the first step must split the surface band"
+
+
 def test_cfl_dt_parity():
     sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND, dt_cfl=True)
     rep = []
