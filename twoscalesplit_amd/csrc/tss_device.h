@@ -203,6 +203,7 @@ struct StepShape {
     int list_parity;          // which list buffer holds the current lists
     int relax_iters;          // max relaxation iterations over the scenes
     int clear_blocks;         // blocks of the cell-table clear (clear_blk)
+    int cap_binds;            // 1: a resident scene has cap_high < 4 cap_low
 };
 
 // Timing points of one step (events recorded before / after each phase):

@@ -234,42 +234,45 @@ __global__ void k_fill_int(int* __restrict__ p, int v, long long n) {
         p[i] = v;
 }
 
-// Start of a step: clear the cell tables of the resident scenes (block b
-// clears the cells of range clear_blk[b]) and per-scene step accumulators.
-__global__ void k_step_begin(DevState d) {
-    const int2 r = d.clear_blk[blockIdx.x];
-    for (int i = r.x + (int)threadIdx.x; i < r.y; i += (int)blockDim.x) {
-        d.cstart[i] = -1;
-        d.cend[i] = -1;
-        d.cflag[2 * i] = 0;
-        d.cflag[2 * i + 1] = 0;
-        d.csurf[i] = 0;
+// Start of a step, one launch: blocks [0, nclear) clear the cell tables of
+// the resident scenes (block b the cells of range clear_blk[b]) and reset
+// the per-scene step accumulators; the other blocks hash (R:1246-1298):
+// key = unified cell index of the particle. The two parts touch disjoint
+// data; the cell tables are next written by k_reorder, after the sort.
+__global__ void k_begin_hash(DevState d, int nclear,
+                             const int* __restrict__ list_l,
+                             const int* __restrict__ list_h,
+                             uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ vals) {
+    if ((int)blockIdx.x < nclear) {
+        const int2 r = d.clear_blk[blockIdx.x];
+        for (int i = r.x + (int)threadIdx.x; i < r.y; i += (int)blockDim.x) {
+            d.cstart[i] = -1;
+            d.cend[i] = -1;
+            d.cflag[2 * i] = 0;
+            d.cflag[2 * i + 1] = 0;
+            d.csurf[i] = 0;
+        }
+        for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < d.S;
+             s += nclear * blockDim.x) {
+            SceneCount& c = d.scnt[s];
+            c.n_split = 0;
+            c.n_promote = 0;
+            c.n_merge = 0;
+            c.n_dissolve = 0;
+            c.cand = 0;
+            c.seg_l0 = 0;
+            c.seg_l1 = 0;
+            c.seg_h0 = 0;
+            c.seg_h1 = 0;
+            c.kl0 = c.kl1 = c.kh0 = c.kh1 = c.sp0 = c.sp1 = 0;
+        }
+        return;
     }
-    GRID_STRIDE(s, d.S) {
-        SceneCount& c = d.scnt[s];
-        c.n_split = 0;
-        c.n_promote = 0;
-        c.n_merge = 0;
-        c.n_dissolve = 0;
-        c.cand = 0;
-        c.seg_l0 = 0;
-        c.seg_l1 = 0;
-        c.seg_h0 = 0;
-        c.seg_h1 = 0;
-        c.kl0 = c.kl1 = c.kh0 = c.kh1 = c.sp0 = c.sp1 = 0;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Hash (R:1246-1298): key = unified cell index of the particle.
-// ---------------------------------------------------------------------------
-__global__ void k_hash(DevState d, const int* __restrict__ list_l,
-                       const int* __restrict__ list_h,
-                       uint32_t* __restrict__ keys,
-                       uint32_t* __restrict__ vals) {
     const int nl = d.counts->n_l;
     const int n = nl + d.counts->n_h;
-    GRID_STRIDE(k, n) {
+    const int stride = (gridDim.x - nclear) * blockDim.x;
+    for (int k = (blockIdx.x - nclear) * blockDim.x + threadIdx.x; k < n; k += stride) {
         int u = k < nl ? list_l[k] : list_h[k - nl];
         bool high = u >= d.CL;
         int s = d.scene_of[low_of(d, u)];
@@ -1632,9 +1635,8 @@ void launch_fill_int(int* p, int value, long long n, void* stream) {
 static int enqueue_grid(const DevState& d, const StepShape& sh,
                         hipStream_t st) {
     const int par = sh.list_parity;
-    k_step_begin<<<sh.clear_blocks, 256, 0, st>>>(d);
-    k_hash<<<sh.grid_elem, 256, 0, st>>>(d, d.list_l[par], d.list_h[par],
-                                         d.keys[0], d.vals[0]);
+    k_begin_hash<<<sh.clear_blocks + sh.grid_elem, 256, 0, st>>>(
+        d, sh.clear_blocks, d.list_l[par], d.list_h[par], d.keys[0], d.vals[0]);
     int cur = 0;
     for (int pass = 0; pass < sh.passes; ++pass) {
         const int shift = kRadixBits * pass;
@@ -1687,7 +1689,9 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     mark(6);
     k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
     k_groups<<<blocks_for(sh.cap_alive, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
-    k_cap<<<sh.S, 256, 0, st>>>(d);
+    // the cap can bind only where cap_high < 4 cap_low: a scene's groups
+    // and candidates never exceed its low ids
+    if (sh.cap_binds) k_cap<<<sh.S, 256, 0, st>>>(d);
     const int ntiles = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
     k_compact_reduce<<<ntiles, kScanThreads, 0, st>>>(d, sh, vals,
                                                       d.list_g[par]);
