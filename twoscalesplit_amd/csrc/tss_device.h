@@ -172,6 +172,9 @@ struct DevState {
     double* cons;             // [S * 10] conservation sums + bounds
     uint8_t* cflag;           // [2 (CCL + CCH)] cflag[2c + s] = 1: state s in cell c
     uint8_t* csurf;           // [CCL + CCH] 1: a free-surface particle in cell c
+    // [CCL] 1: a free-surface particle (either level) in this low cell or
+    // one of the eight around it
+    uint8_t* cband;
     float4* rtmp;             // [4 CL] relaxation scratch (x', v') per target;
                               // allocated when a scene enables relaxation
 };

@@ -265,7 +265,7 @@ struct tss_batch {
         b += CL * 2;
         b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) + 10 * 8 + 12);
         b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
-        b += 3 * (ds.cells_low + ds.cells_high);  // cell state and surface flags
+        b += 3 * (ds.cells_low + ds.cells_high) + ds.cells_low;  // cell state and surface flags
         b += 4 * (ds.cells_low + ds.cells_high);  // cell -> scene
         b += 8 * ((ds.cells_low + ds.cells_high) / tss::kClearChunk + 2LL * ds.max_scenes + 1);
         b += 16 * 4 * CL;                         // relaxation scratch (when used)
@@ -348,6 +348,7 @@ struct tss_batch {
         d.cons = dalloc<double>((size_t)max_scenes * 10, owned);
         d.cflag = dalloc<uint8_t>(2 * (ccl + cch), owned);
         d.csurf = dalloc<uint8_t>(ccl + cch, owned);
+        d.cband = dalloc<uint8_t>(ccl, owned);
         // initial contents
         HIP_CHECK(hipMemsetAsync(d.scenes, 0, sizeof(tss::SceneDev) * max_scenes, stream));
         HIP_CHECK(hipMemsetAsync(d.scnt, 0, sizeof(tss::SceneCount) * max_scenes, stream));
@@ -356,6 +357,7 @@ struct tss_batch {
         HIP_CHECK(hipMemsetAsync(d.red, 0, sizeof(double) * tss::kRedFields * red_cap, stream));
         HIP_CHECK(hipMemsetAsync(d.cflag, 0, 2 * (size_t)(ccl + cch), stream));
         HIP_CHECK(hipMemsetAsync(d.csurf, 0, (size_t)(ccl + cch), stream));
+        HIP_CHECK(hipMemsetAsync(d.cband, 0, (size_t)ccl, stream));
         tss::launch_fill_int(d.scene_of, 0, CL, stream);
         HIP_CHECK(hipMemsetAsync(d.state, 0, (size_t)U, stream));
         HIP_CHECK(hipMemsetAsync(d.pos, 0, sizeof(float2) * U, stream));

@@ -252,6 +252,7 @@ __global__ void k_begin_hash(DevState d, int nclear,
             d.cflag[2 * i] = 0;
             d.cflag[2 * i + 1] = 0;
             d.csurf[i] = 0;
+            if (i < d.CCL) d.cband[i] = 0;
         }
         for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < d.S;
              s += nclear * blockDim.x) {
@@ -753,6 +754,18 @@ __global__ __launch_bounds__(256) void k_reduce(DevState d,
 // density pass) are read.
 __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
                         float r, float t) {
+    // coarse test first: a surface particle within r of x lies in a low cell
+    // at most one cell outside x's low box of radius r (rounding included)
+    // and marks its own low cell and the eight around it (cband), so a box
+    // without a mark holds no such particle
+    {
+        const Box b = scan_box(sc, false, x, r);
+        bool any = false;
+        for (int cy = b.y0; cy <= b.y1 && !any; ++cy)
+            for (int cx = b.x0; cx <= b.x1 && !any; ++cx)
+                any = d.cband[sc.cbase_l + cy * sc.dlx + cx] != 0;
+        if (!any) return false;
+    }
     for (int lv = 0; lv < 2; ++lv) {
         const bool high = lv == 1;
         const Box b = scan_box(sc, high, x, r);
