@@ -91,7 +91,7 @@ struct SceneCount {
     // compaction prefixes at the scene's segment ends (kept low, kept high,
     // admitted splits): counts = end - begin, no hot atomics
     int kl0, kl1, kh0, kh1, sp0, sp1;
-    int pad0;
+    int prom;                 // 1: a listed high particle is promoted (state 0)
     unsigned long long psteps;  // particle-steps advanced since admission
 };
 

@@ -267,6 +267,7 @@ __global__ void k_begin_hash(DevState d, int nclear,
             c.seg_h0 = 0;
             c.seg_h1 = 0;
             c.kl0 = c.kl1 = c.kh0 = c.kh1 = c.sp0 = c.sp1 = 0;
+            c.prom = 0;
         }
         return;
     }
@@ -503,18 +504,21 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
         if (k == n - 1 || keys[k + 1] != key) d.cend[key] = k + 1;
         // states present per cell (every particle stores the same 1 for its
         // state): lets the scans skip rows whose weights are all zero
-        d.cflag[2 * (size_t)key + (d.state[u] & 1u)] = 1;
+        const uint8_t st = d.state[u];
+        d.cflag[2 * (size_t)key + (st & 1u)] = 1;
         const float2 x = d.pos[u];
         const float2 v = d.vel[u];
         d.spos[k] = x;
         d.sxv[k] = make_float4(x.x, x.y, v.x, v.y);
-        d.sstate[k] = d.state[u];
+        d.sstate[k] = st;
         d.sidx[u] = k;
         // the scene owns the cell (cells of a scene are one range)
         const int s = d.cscene[key];
         d.sscene[k] = s;
         // scene segments per level: low part [0, nl), high part [nl, n)
         const bool high = k >= nl;
+        // promoted children present: the low level's cross passes run
+        if (high && !(st & 1u)) d.scnt[s].prom = 1;
         const int first = high ? nl : 0;
         const int last = high ? n - 1 : nl - 1;
         const bool starts = k == first || d.cscene[keys[k - 1]] != s;
