@@ -92,6 +92,8 @@ struct SceneCount {
     // admitted splits): counts = end - begin, no hot atomics
     int kl0, kl1, kh0, kh1, sp0, sp1;
     int prom;                 // 1: a listed high particle is promoted (state 0)
+    int act;                  // 1: a listed low particle is active (state 0)
+    int pad1;
     unsigned long long psteps;  // particle-steps advanced since admission
 };
 

@@ -268,6 +268,7 @@ __global__ void k_begin_hash(DevState d, int nclear,
             c.seg_h1 = 0;
             c.kl0 = c.kl1 = c.kh0 = c.kh1 = c.sp0 = c.sp1 = 0;
             c.prom = 0;
+            c.act = 0;
         }
         return;
     }
@@ -517,8 +518,12 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
         d.sscene[k] = s;
         // scene segments per level: low part [0, nl), high part [nl, n)
         const bool high = k >= nl;
-        // promoted children present: the low level's cross passes run
-        if (high && !(st & 1u)) d.scnt[s].prom = 1;
+        // state-0 particles present: promoted children make the low
+        // level's cross passes run, active low particles the high level's
+        if (!(st & 1u)) {
+            if (high) d.scnt[s].prom = 1;
+            else d.scnt[s].act = 1;
+        }
         const int first = high ? nl : 0;
         const int last = high ? n - 1 : nl - 1;
         const bool starts = k == first || d.cscene[keys[k - 1]] != s;
