@@ -825,18 +825,22 @@ __global__ void k_split_cand(DevState d, const uint32_t* __restrict__ vals) {
 }
 
 // R's adjustPosition (R:1007-1022): r and r/dist in double (the reference's
-// min(float, double)); a coincident child is left in place.
-__device__ __forceinline__ float2 clamp_child(float2 ph, float2 pl, float m_l,
-                                              float rho) {
+// min(float, double)); a coincident child is left in place. rmax =
+// 0.75 sqrt(m_l / (rho pi)) in double, the same for a group's four
+// children (clamp_radius). Where dist < rmax, r / dist = dist / dist = 1
+// exactly, so the division is taken only for a child that is clamped.
+__device__ __forceinline__ double clamp_radius(float m_l, float rho) {
+    return 0.75 * sqrt((double)m_l / ((double)rho * 3.14159265358979323846));
+}
+
+__device__ __forceinline__ float2 clamp_child(float2 ph, float2 pl, double rmax) {
     float dx = ph.x - pl.x;
     float dy = ph.y - pl.y;
     float a = dx * dx;
     float b = dy * dy;
     float dist = sqrtf(a + b);
     if (!(dist > 0.0f)) return ph;
-    double rmax = 0.75 * sqrt((double)m_l / ((double)rho * 3.14159265358979323846));
-    double r = (double)dist < rmax ? (double)dist : rmax;
-    double t = r / (double)dist;
+    const double t = (double)dist < rmax ? 1.0 : rmax / (double)dist;
     return make_float2((float)((double)pl.x + t * (double)dx),
                        (float)((double)pl.y + t * (double)dy));
 }
@@ -885,9 +889,9 @@ __global__ void k_groups(DevState d, const int* __restrict__ list_g) {
                     atomicAdd(&d.scnt[s].n_dissolve, 1);
                 } else {
                     const float2 pl = d.pos[p];
-                    const float rho = d.rho[p];
+                    const double rmax = clamp_radius(sc.m_l, d.rho[p]);
                     for (int k = 0; k < 4; ++k)
-                        d.pos[c0 + k] = clamp_child(d.pos[c0 + k], pl, sc.m_l, rho);
+                        d.pos[c0 + k] = clamp_child(d.pos[c0 + k], pl, rmax);
                 }
             }
         } else if (st == kDeleted && sc.mode == 2) {
