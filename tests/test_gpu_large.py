@@ -1,6 +1,12 @@
-"""Full-size invariants of the large forced two-scale dam break
-(1,414 x 707 = 999,698 low particles, every one split at the first step into
-3,998,792 high children), where the oracle is too slow to run beside it:
+"""Full-size checks of the large forced two-scale dam break (1,414 x 707 =
+999,698 low particles, every one split at the first step into 3,998,792
+high children).
+
+One step of the full state against the oracle (OpenMP, seconds on the host):
+grid bit-exact, rho / p / a within the parity tolerances, lists, states,
+positions, velocities and dt bit-exact.
+
+Invariants over several steps:
 
 * the lists are duplicate-free sets of valid ids; n_high = 4 n_groups and the
   listed children are exactly the children of the listed groups;
@@ -14,14 +20,30 @@
 """
 from __future__ import annotations
 
+import json
 import math
 
 import numpy as np
 import pytest
 
+from parity import check_step
 from twoscalesplit_amd import Solver, scenes
 
 pytestmark = pytest.mark.gpu
+
+
+def test_large_forced_split_step_parity():
+    sc = scenes.c3_large(forced=True)
+    print("[case] large forced split: one full-size step against the oracle", flush=True)
+    sol = Solver(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    try:
+        sol.step(2)  # the split happens at step 1; step 3 advances 5 M particles
+        rep = []
+        check_step(sol, sc.cfg, sc.bnd_xy, rep)
+        print(json.dumps(rep), flush=True)
+        assert rep[0]["n"] == (sc.n_low, 4 * sc.n_low)
+    finally:
+        sol.close()
 
 
 def _check_grid(sol, cnt, level):
