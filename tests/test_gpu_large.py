@@ -32,16 +32,23 @@ from twoscalesplit_amd import Solver, scenes
 pytestmark = pytest.mark.gpu
 
 
-def test_large_forced_split_step_parity():
-    sc = scenes.c3_large(forced=True)
-    print("[case] large forced split: one full-size step against the oracle", flush=True)
+@pytest.mark.parametrize("forced", [True, False])
+def test_large_step_parity(forced):
+    # forced: every particle split at step 1, so step 3 advances 5 M
+    # particles; natural: the surface-band rules at full size
+    sc = scenes.c3_large(forced=forced)
+    print("[case] large", "forced split" if forced else "surface band",
+          ": one full-size step against the oracle", flush=True)
     sol = Solver(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
     try:
-        sol.step(2)  # the split happens at step 1; step 3 advances 5 M particles
+        sol.step(2)
         rep = []
         check_step(sol, sc.cfg, sc.bnd_xy, rep)
         print(json.dumps(rep), flush=True)
-        assert rep[0]["n"] == (sc.n_low, 4 * sc.n_low)
+        if forced:
+            assert rep[0]["n"] == (sc.n_low, 4 * sc.n_low)
+        else:
+            assert rep[0]["n"][1] > 0, "the band must hold high particles"
     finally:
         sol.close()
 
