@@ -77,23 +77,42 @@ def _traffic(kernel: str) -> dict | None:
     return None
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(snap: dict, sc, steps: int, threads: int) -> dict:
     """The oracle's FP32 transcription of the reference kernels (OpenMP)
-    on the GPU state of the same workload."""
+    on the GPU state of the same workload, on `threads` host threads and
+    (one step) on one."""
     from oracle import binding as ob
-    o = ob.Scene(ob.params_from_config(sc.cfg), sc.cap_low, sc.bnd_xy)
-    o.load_snapshot(snap)
-    n = o.s.n_l + o.s.n_h
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        o.step(32, threads)
-    dt = time.perf_counter() - t0
-    return {"value": steps * n / dt, "unit": "particle-steps/s",
+
+    def rate(nsteps: int, nthreads: int) -> tuple[float, float, int]:
+        o = ob.Scene(ob.params_from_config(sc.cfg), sc.cap_low, sc.bnd_xy)
+        o.load_snapshot(snap)
+        n = o.s.n_l + o.s.n_h
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            o.step(32, nthreads)
+        dt = time.perf_counter() - t0
+        return nsteps * n / dt, dt, n
+
+    value, dt, n = rate(steps, threads)
+    one, dt1, _ = rate(1, 1)
+    return {"value": value, "unit": "particle-steps/s",
             "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the same {n}-particle state "
                       f"(taken from the GPU after warmup), oracle FP32 "
                       f"transcription of the reference kernels, OpenMP",
-            "seconds": round(dt, 3)}
+            "seconds": round(dt, 3),
+            "value_1_thread": one, "seconds_1_thread": round(dt1, 3),
+            "cpu_model": _cpu_model()}
 
 
 def run_batched(args, rule: str) -> dict:
