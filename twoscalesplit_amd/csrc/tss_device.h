@@ -104,6 +104,12 @@ struct DevCounts {
     // children injected by the last step's splits: the last n_fresh entries
     // of the high list (new children are appended after the kept ones)
     int n_fresh, n_fresh_next;
+    // 1: the step has a split candidate, promotion, merge or dissolve, or
+    // the host changed the state since the last step; 0: every listed
+    // entry is kept as it is (the compaction needs no flags). Reset to
+    // events_floor at the end of every step (1 once raw device pointers
+    // were handed out: the state may change behind the library's back).
+    int events, events_floor;
 };
 
 // Device buffers of a batch (all pointers are device memory).

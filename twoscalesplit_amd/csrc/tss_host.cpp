@@ -495,8 +495,18 @@ struct tss_batch {
         return c;
     }
 
+    // Host changes to lists or counts: the next step's compaction takes the
+    // full path (events = 1).
     void write_counts(const tss::DevCounts& c) {
-        HIP_CHECK(hipMemcpyAsync(d.counts, &c, sizeof(c), hipMemcpyHostToDevice, stream));
+        tss::DevCounts w = c;
+        w.events = 1;
+        HIP_CHECK(hipMemcpyAsync(d.counts, &w, sizeof(w), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    void mark_host_change() {
+        const int one = 1;
+        HIP_CHECK(hipMemcpyAsync(&d.counts->events, &one, sizeof(int), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -1183,6 +1193,7 @@ struct tss_batch {
             require(bytes <= v.elem * v.count, "write exceeds field size");
             HIP_CHECK(hipMemcpyAsync(v.dev, src, bytes, hipMemcpyHostToDevice, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
+            mark_host_change();
             invalidate_graphs();
             return;
         }
@@ -1218,6 +1229,7 @@ struct tss_batch {
             HIP_CHECK(hipMemcpyAsync(v.dev, buf.data(), bytes, hipMemcpyHostToDevice, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
         }
+        mark_host_change();
     }
 
     // ---- frame export ------------------------------------------------------
@@ -1655,9 +1667,14 @@ struct tss_batch {
         HIP_CHECK(hipGetLastError());
     }
 
+    // A raw device pointer lets the caller change the state between steps
+    // unseen: from now on every step's compaction takes the full path.
     void* device_ptr(int sid, int field) {
         FieldView v = view(sid, field);
         require(v.kind == 0, "field has no plain device view");
+        const int one[2] = {1, 1};
+        HIP_CHECK(hipMemcpyAsync(&d.counts->events, one, sizeof(one), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
         return v.dev;
     }
 };

@@ -816,10 +816,13 @@ __global__ void k_split_cand(DevState d, const uint32_t* __restrict__ vals) {
             const uint64_t cm = __ballot(cand != 0);
             const uint64_t act = __ballot(true);
             const int lane = threadIdx.x & 63;
-            if (cm && lane == __ffsll((long long)act) - 1)
+            if (cm && lane == __ffsll((long long)act) - 1) {
                 atomicAdd(&d.scnt[s0].cand, (int)__popcll(cm));
+                d.counts->events = 1;
+            }
         } else if (cand) {
             atomicAdd(&d.scnt[s].cand, 1);
+            d.counts->events = 1;
         }
     }
 }
@@ -872,6 +875,7 @@ __global__ void k_groups(DevState d, const int* __restrict__ list_g) {
                 d.state[p] = kDeleted;
                 for (int k = 0; k < 4; ++k) d.state[c0 + k] = 0;
                 atomicAdd(&d.scnt[s].n_promote, 1);
+                d.counts->events = 1;
             } else {
                 bool keep = true;
                 if (sc.mode == 2) {
@@ -887,6 +891,7 @@ __global__ void k_groups(DevState d, const int* __restrict__ list_g) {
                     d.state[p] = kActive;
                     d.age[p] = 0;
                     atomicAdd(&d.scnt[s].n_dissolve, 1);
+                    d.counts->events = 1;
                 } else {
                     const float2 pl = d.pos[p];
                     const double rmax = clamp_radius(sc.m_l, d.rho[p]);
@@ -915,6 +920,7 @@ __global__ void k_groups(DevState d, const int* __restrict__ list_g) {
                 d.age[p] = 0;
                 for (int k = 0; k < 4; ++k) d.state[c0 + k] = 1;
                 atomicAdd(&d.scnt[s].n_merge, 1);
+                d.counts->events = 1;
             }
         }
         d.merged[p] = merged;
@@ -1020,10 +1026,13 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_reduce(
     int seg, t;
     tile_seg(sh, blockIdx.x, seg, t);
     // a tile past its segment's live entries counts nothing (the grid is
-    // sized by capacity): one write, and the whole block exits
-    if (t * kScanTile >= (seg == 0 ? nl : seg == 1 ? nh : ng)) {
+    // sized by capacity): one write, and the whole block exits. A step
+    // without events keeps every live entry and splits or re-lists none:
+    // the counts follow from the segment sizes alone
+    const int nseg = seg == 0 ? nl : seg == 1 ? nh : ng;
+    if (t * kScanTile >= nseg || !d.counts->events) {
         if (threadIdx.x == 0) {
-            d.tile_sums[2 * blockIdx.x] = 0;
+            d.tile_sums[2 * blockIdx.x] = max(0, min(kScanTile, nseg - t * kScanTile));
             d.tile_sums[2 * blockIdx.x + 1] = 0;
         }
         return;
@@ -1128,9 +1137,13 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
     int run1 = d.tile_base[2 * blockIdx.x + 1];
     const int keepL = d.seg_base[3], keepH = d.seg_base[4],
               keepG = d.seg_base[5];
+    const int nseg = seg == 0 ? nl : seg == 1 ? nh : ng;
+    const bool events = d.counts->events != 0;
     for (int q = 0; q < kScanItems; ++q) {
         const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
-        const Flags fq = entry_flags(d, vals, list_g, seg, i, nl, nh, ng);
+        // without events every live entry is kept and none splits or merges
+        const Flags fq = events ? entry_flags(d, vals, list_g, seg, i, nl, nh, ng)
+                                : Flags{i < nseg ? 1 : 0, 0};
         const uint64_t m0 = __ballot(fq.p0), m1 = __ballot(fq.p1);
         if (lane == 0) {
             wsum[0][wv] = __popcll(m0);
@@ -1283,6 +1296,7 @@ __global__ __launch_bounds__(256) void k_finalize(DevState d) {
         g.n_h = g.n_h_next;
         g.n_g = g.n_g_next;
         g.n_fresh = g.n_fresh_next;
+        g.events = g.events_floor;
     }
 }
 
