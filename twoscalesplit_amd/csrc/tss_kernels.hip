@@ -766,13 +766,21 @@ __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
     // coarse test first: a surface particle within r of x lies in a low cell
     // at most one cell outside x's low box of radius r (rounding included)
     // and marks its own low cell and the eight around it (cband), so a box
-    // without a mark holds no such particle
+    // without a mark holds no such particle. Flags are read eight cells of
+    // a row at a time, the loads issued together (indices clamped into the
+    // box: a repeated cell changes no OR).
     {
         const Box b = scan_box(sc, false, x, r);
         bool any = false;
-        for (int cy = b.y0; cy <= b.y1 && !any; ++cy)
-            for (int cx = b.x0; cx <= b.x1 && !any; ++cx)
-                any = d.cband[sc.cbase_l + cy * sc.dlx + cx] != 0;
+        for (int cy = b.y0; cy <= b.y1 && !any; ++cy) {
+            const int row = sc.cbase_l + cy * sc.dlx;
+            for (int c0 = b.x0; c0 <= b.x1 && !any; c0 += 8) {
+                uint32_t f = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) f |= d.cband[row + min(c0 + k, b.x1)];
+                any = f != 0;
+            }
+        }
         if (!any) return false;
     }
     for (int lv = 0; lv < 2; ++lv) {
@@ -781,12 +789,20 @@ __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
         const int base = high ? sc.cbase_h : sc.cbase_l;
         const int dimx = high ? sc.dhx : sc.dlx;
         for (int cy = b.y0; cy <= b.y1; ++cy) {
-            for (int cx = b.x0; cx <= b.x1; ++cx) {
-                const int c = base + cy * dimx + cx;
-                if (!d.csurf[c]) continue;
-                const int en = d.cend[c];
-                for (int j = d.cstart[c]; j < en; ++j)
-                    if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
+            const int row = base + cy * dimx;
+            for (int c0 = b.x0; c0 <= b.x1; c0 += 8) {
+                // the row's surface-cell flags, eight at a time
+                uint32_t m = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    m |= (c0 + k <= b.x1 && d.csurf[row + min(c0 + k, b.x1)]) ? (1u << k) : 0u;
+                while (m) {
+                    const int c = row + c0 + __builtin_ctz(m);
+                    m &= m - 1u;
+                    const int en = d.cend[c];
+                    for (int j = d.cstart[c]; j < en; ++j)
+                        if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
+                }
             }
         }
     }
