@@ -1078,15 +1078,28 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_reduce(
 }
 
 // Single block: exclusive scan of the tile sums, segment bases, new counts.
+// Only tiles holding live entries are scanned (the others count zero and
+// the scatter never reads their bases): virtual tile v runs over the live
+// tiles of the low, high and group segments in that order.
 __global__ __launch_bounds__(1024) void k_compact_top(DevState d, StepShape sh) {
     __shared__ int2 part[1024];
-    const int ntiles = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
-    const int per = (ntiles + 1023) / 1024;
+    __shared__ int2 seg_pre[2];  // prefixes where the high / group tiles start
+    const int th = sh.scan_tiles_l, tg = sh.scan_tiles_l + sh.scan_tiles_h;
+    const DevCounts* c = d.counts;
+    const int ll = min((c->n_l + kScanTile - 1) / kScanTile, sh.scan_tiles_l);
+    const int lh = min((c->n_h + kScanTile - 1) / kScanTile, sh.scan_tiles_h);
+    const int lg = min((c->n_g + kScanTile - 1) / kScanTile, sh.scan_tiles_g);
+    const int nv = ll + lh + lg;
+    auto real = [&](int v) {
+        return v < ll ? v : (v < ll + lh ? th + (v - ll) : tg + (v - ll - lh));
+    };
+    const int per = (nv + 1023) / 1024;
     const int t0 = threadIdx.x * per;
     int a = 0, b = 0;
     for (int i = 0; i < per; ++i) {
-        int t = t0 + i;
-        if (t < ntiles) {
+        const int v = t0 + i;
+        if (v < nv) {
+            const int t = real(v);
             a += d.tile_sums[2 * t];
             b += d.tile_sums[2 * t + 1];
         }
@@ -1094,18 +1107,28 @@ __global__ __launch_bounds__(1024) void k_compact_top(DevState d, StepShape sh) 
     part[threadIdx.x] = make_int2(a, b);
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-        int2 v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off]
+        int2 w = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off]
                                               : make_int2(0, 0);
         __syncthreads();
-        part[threadIdx.x].x += v.x;
-        part[threadIdx.x].y += v.y;
+        part[threadIdx.x].x += w.x;
+        part[threadIdx.x].y += w.y;
         __syncthreads();
     }
+    const int tot0 = part[1023].x, tot1 = part[1023].y;
+    if (threadIdx.x == 0) {
+        // a segment without live tiles starts at the total
+        seg_pre[0] = make_int2(tot0, tot1);
+        seg_pre[1] = make_int2(tot0, tot1);
+    }
+    __syncthreads();
     int ra = part[threadIdx.x].x - a, rb = part[threadIdx.x].y - b;
     for (int i = 0; i < per; ++i) {
-        int t = t0 + i;
-        if (t < ntiles) {
-            int va = d.tile_sums[2 * t], vb = d.tile_sums[2 * t + 1];
+        const int v = t0 + i;
+        if (v < nv) {
+            const int t = real(v);
+            const int va = d.tile_sums[2 * t], vb = d.tile_sums[2 * t + 1];
+            if (v == ll) seg_pre[0] = make_int2(ra, rb);
+            if (v == ll + lh) seg_pre[1] = make_int2(ra, rb);
             d.tile_base[2 * t] = ra;
             d.tile_base[2 * t + 1] = rb;
             ra += va;
@@ -1114,10 +1137,8 @@ __global__ __launch_bounds__(1024) void k_compact_top(DevState d, StepShape sh) 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int th = sh.scan_tiles_l, tg = sh.scan_tiles_l + sh.scan_tiles_h;
-        const int tot0 = part[1023].x, tot1 = part[1023].y;
-        const int b0h = d.tile_base[2 * th], b1h = d.tile_base[2 * th + 1];
-        const int b0g = d.tile_base[2 * tg], b1g = d.tile_base[2 * tg + 1];
+        const int b0h = seg_pre[0].x, b1h = seg_pre[0].y;
+        const int b0g = seg_pre[1].x, b1g = seg_pre[1].y;
         const int keepL = b0h, nsplit = b1h;
         const int keepH = b0g - b0h;
         const int keepG = tot0 - b0g, nmerge = tot1 - b1g;
