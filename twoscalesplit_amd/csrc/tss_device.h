@@ -215,6 +215,7 @@ struct StepShape {
     int relax_iters;          // max relaxation iterations over the scenes
     int clear_blocks;         // blocks of the cell-table clear (clear_blk)
     int cap_binds;            // 1: a resident scene has cap_high < 4 cap_low
+    int band_any;             // 1: a resident scene uses the surface band rule
 };
 
 // Timing points of one step (events recorded before / after each phase):

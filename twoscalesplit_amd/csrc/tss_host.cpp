@@ -456,8 +456,11 @@ struct tss_batch {
         sh.scan_tiles_h = (int)((4 * cap + tss::kScanTile - 1) / tss::kScanTile);
         sh.scan_tiles_g = sh.scan_tiles_l;
         sh.cap_binds = 0;
-        for (const SceneHost& s : scenes)
+        sh.band_any = 0;
+        for (const SceneHost& s : scenes) {
             if (s.alive && (long long)s.cfg.cap_high < 4LL * s.lcap) sh.cap_binds = 1;
+            if (s.alive && s.cfg.mode == 2) sh.band_any = 1;
+        }
         long long maxkey = 1;
         for (auto& s : scenes)
             if (s.alive) maxkey = std::max<long long>(maxkey, (long long)s.cbh + s.nch);

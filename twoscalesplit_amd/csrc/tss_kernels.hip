@@ -866,78 +866,86 @@ __device__ __forceinline__ float2 clamp_child(float2 ph, float2 pl, double rmax)
 
 // Existing groups: age, promotion (dormant test of R:1053-1095, enabled by
 // the method), child clamp (R:1097-1172), merge decision and application.
+// LANES lanes per group: under the band rule the merge test scans the
+// neighbourhoods of up to four children, one child per lane; the lanes of a
+// group sit in one wave and OR their results in a ballot, and lane 0
+// applies the decision. LANES = 1 tests the children in turn.
+template <int LANES>
 __global__ void k_groups(DevState d, const int* __restrict__ list_g) {
+    static_assert(LANES == 1 || LANES == 4, "one lane or one per child");
     const int ng = d.counts->n_g;
-    GRID_STRIDE(q, ng) {
+    GRID_STRIDE(i, LANES * ng) {
+        const int q = i / LANES, kl = i % LANES;
         const int p = list_g[q];
         const int s = d.scene_of[p];
         const SceneDev& sc = d.scenes[s];
         const int c0 = d.CL + 4 * p;
         const uint8_t st = d.state[p];
-        // merged: 1 = promoted group merged back (parent re-listed),
-        //         2 = transient group dissolved (parent already listed)
-        uint8_t merged = 0;
+        int age = 0;
+        bool promote = false;
         if (st == kTransient) {
-            int age = d.age[p] + 1;
-            d.age[p] = age;
-            bool promote = false;
+            age = d.age[p] + 1;
             if (sc.mode == 2 && sc.pmax > 0) {
                 float fluc = 0.0f;
                 for (int k = 0; k < 4; ++k) fluc += fabsf(sc.rho0 - d.rho[c0 + k]);
                 fluc = fluc / (4.0f * sc.rho0);
                 promote = (age >= sc.pmin && fluc <= sc.pfluc) || age >= sc.pmax;
             }
+        }
+        // merge test (band rule): a child within the merge radius of a
+        // surface particle keeps the group
+        const bool test = sc.mode == 2 && ((st == kTransient && !promote) || st == kDeleted);
+        bool keep = false;
+        if (LANES == 1) {
+            for (int k = 0; k < 4 && test && !keep; ++k)
+                keep = in_band(d, sc, d.spos[d.sidx[c0 + k]], sc.r_merge, sc.t_merge);
+        } else {
+            const bool mine = test && in_band(d, sc, d.spos[d.sidx[c0 + kl]],
+                                              sc.r_merge, sc.t_merge);
+            const uint64_t bal = __ballot(mine);
+            const int g0 = (threadIdx.x & 63) & ~(LANES - 1);
+            keep = ((bal >> g0) & ((1ull << LANES) - 1ull)) != 0ull;
+            if (kl != 0) continue;
+        }
+        // merged: 1 = promoted group merged back (parent re-listed),
+        //         2 = transient group dissolved (parent already listed)
+        uint8_t merged = 0;
+        if (st == kTransient) {
+            d.age[p] = age;
             if (promote) {
                 d.state[p] = kDeleted;
                 for (int k = 0; k < 4; ++k) d.state[c0 + k] = 0;
                 atomicAdd(&d.scnt[s].n_promote, 1);
                 d.counts->events = 1;
-            } else {
-                bool keep = true;
-                if (sc.mode == 2) {
-                    keep = false;
-                    for (int k = 0; k < 4 && !keep; ++k) {
-                        float2 x = d.spos[d.sidx[c0 + k]];
-                        keep = in_band(d, sc, x, sc.r_merge, sc.t_merge);
-                    }
-                }
-                if (!keep) {
-                    // dissolve: children leave the lists, the parent stays
-                    merged = 2;
-                    d.state[p] = kActive;
-                    d.age[p] = 0;
-                    atomicAdd(&d.scnt[s].n_dissolve, 1);
-                    d.counts->events = 1;
-                } else {
-                    const float2 pl = d.pos[p];
-                    const double rmax = clamp_radius(sc.m_l, d.rho[p]);
-                    for (int k = 0; k < 4; ++k)
-                        d.pos[c0 + k] = clamp_child(d.pos[c0 + k], pl, rmax);
-                }
-            }
-        } else if (st == kDeleted && sc.mode == 2) {
-            bool keep = false;
-            for (int k = 0; k < 4 && !keep; ++k) {
-                float2 x = d.spos[d.sidx[c0 + k]];
-                keep = in_band(d, sc, x, sc.r_merge, sc.t_merge);
-            }
-            if (!keep) {
-                merged = 1;
-                float2 c[4], v[4];
-                for (int k = 0; k < 4; ++k) {
-                    c[k] = d.pos[c0 + k];
-                    v[k] = d.vel[c0 + k];
-                }
-                d.pos[p] = make_float2(((c[0].x + c[1].x) + (c[2].x + c[3].x)) * 0.25f,
-                                       ((c[0].y + c[1].y) + (c[2].y + c[3].y)) * 0.25f);
-                d.vel[p] = make_float2(((v[0].x + v[1].x) + (v[2].x + v[3].x)) * 0.25f,
-                                       ((v[0].y + v[1].y) + (v[2].y + v[3].y)) * 0.25f);
+            } else if (sc.mode == 2 && !keep) {
+                // dissolve: children leave the lists, the parent stays
+                merged = 2;
                 d.state[p] = kActive;
                 d.age[p] = 0;
-                for (int k = 0; k < 4; ++k) d.state[c0 + k] = 1;
-                atomicAdd(&d.scnt[s].n_merge, 1);
+                atomicAdd(&d.scnt[s].n_dissolve, 1);
                 d.counts->events = 1;
+            } else {
+                const float2 pl = d.pos[p];
+                const double rmax = clamp_radius(sc.m_l, d.rho[p]);
+                for (int k = 0; k < 4; ++k)
+                    d.pos[c0 + k] = clamp_child(d.pos[c0 + k], pl, rmax);
             }
+        } else if (st == kDeleted && sc.mode == 2 && !keep) {
+            merged = 1;
+            float2 c[4], v[4];
+            for (int k = 0; k < 4; ++k) {
+                c[k] = d.pos[c0 + k];
+                v[k] = d.vel[c0 + k];
+            }
+            d.pos[p] = make_float2(((c[0].x + c[1].x) + (c[2].x + c[3].x)) * 0.25f,
+                                   ((c[0].y + c[1].y) + (c[2].y + c[3].y)) * 0.25f);
+            d.vel[p] = make_float2(((v[0].x + v[1].x) + (v[2].x + v[3].x)) * 0.25f,
+                                   ((v[0].y + v[1].y) + (v[2].y + v[3].y)) * 0.25f);
+            d.state[p] = kActive;
+            d.age[p] = 0;
+            for (int k = 0; k < 4; ++k) d.state[c0 + k] = 1;
+            atomicAdd(&d.scnt[s].n_merge, 1);
+            d.counts->events = 1;
         }
         d.merged[p] = merged;
     }
@@ -1765,7 +1773,12 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     k_reduce<<<sh.red_slots, 256, 0, st>>>(d, vals);
     mark(6);
     k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
-    k_groups<<<blocks_for(sh.cap_alive, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
+    // few groups and long band tests: a lane per child, small blocks to
+    // spread them over the CUs
+    if (sh.band_any)
+        k_groups<4><<<blocks_for(4 * sh.cap_alive, 64, 16384), 64, 0, st>>>(d, d.list_g[par]);
+    else
+        k_groups<1><<<blocks_for(sh.cap_alive, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
     // the cap can bind only where cap_high < 4 cap_low: a scene's groups
     // and candidates never exceed its low ids
     if (sh.cap_binds) k_cap<<<sh.S, 256, 0, st>>>(d);
