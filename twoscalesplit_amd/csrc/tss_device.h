@@ -39,6 +39,9 @@ constexpr int kRedChunk = 4096;
 // Spare entries past the capacity of the cell-sorted position and state
 // arrays: the candidate fill reads up to 31 entries beyond a run's end.
 constexpr int kSortedPad = 32;
+// Bytes of padding past the last cell of the csurf and cband flag tables
+// (read as aligned 4-byte words, up to 11 bytes past a row chunk's start).
+constexpr int kFlagPad = 16;
 // mass, px, py, ke, pe, max|v|^2, sum|px|, sum|py|, sum|pe|
 constexpr int kRedFields = 9;
 

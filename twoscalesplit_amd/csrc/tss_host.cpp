@@ -266,6 +266,7 @@ struct tss_batch {
         b += (long long)ds.max_scenes * (sizeof(tss::SceneDev) + sizeof(tss::SceneCount) + 10 * 8 + 12);
         b += (U / tss::kRedChunk + ds.max_scenes + 2) * (tss::kRedFields * 8 + 4);
         b += 3 * (ds.cells_low + ds.cells_high) + ds.cells_low;  // cell state and surface flags
+        b += 2 * tss::kFlagPad;
         b += 4 * (ds.cells_low + ds.cells_high);  // cell -> scene
         b += 8 * ((ds.cells_low + ds.cells_high) / tss::kClearChunk + 2LL * ds.max_scenes + 1);
         b += 16 * 4 * CL;                         // relaxation scratch (when used)
@@ -347,8 +348,10 @@ struct tss_batch {
         d.red_n = dalloc<int>(max_scenes, owned);
         d.cons = dalloc<double>((size_t)max_scenes * 10, owned);
         d.cflag = dalloc<uint8_t>(2 * (ccl + cch), owned);
-        d.csurf = dalloc<uint8_t>(ccl + cch, owned);
-        d.cband = dalloc<uint8_t>(ccl, owned);
+        // the band tests read these flags as aligned words: padded past
+        // the last cell
+        d.csurf = dalloc<uint8_t>(ccl + cch + tss::kFlagPad, owned);
+        d.cband = dalloc<uint8_t>(ccl + tss::kFlagPad, owned);
         // initial contents
         HIP_CHECK(hipMemsetAsync(d.scenes, 0, sizeof(tss::SceneDev) * max_scenes, stream));
         HIP_CHECK(hipMemsetAsync(d.scnt, 0, sizeof(tss::SceneCount) * max_scenes, stream));
@@ -356,8 +359,8 @@ struct tss_batch {
         HIP_CHECK(hipMemsetAsync(d.cons, 0, sizeof(double) * 10 * max_scenes, stream));
         HIP_CHECK(hipMemsetAsync(d.red, 0, sizeof(double) * tss::kRedFields * red_cap, stream));
         HIP_CHECK(hipMemsetAsync(d.cflag, 0, 2 * (size_t)(ccl + cch), stream));
-        HIP_CHECK(hipMemsetAsync(d.csurf, 0, (size_t)(ccl + cch), stream));
-        HIP_CHECK(hipMemsetAsync(d.cband, 0, (size_t)ccl, stream));
+        HIP_CHECK(hipMemsetAsync(d.csurf, 0, (size_t)(ccl + cch + tss::kFlagPad), stream));
+        HIP_CHECK(hipMemsetAsync(d.cband, 0, (size_t)(ccl + tss::kFlagPad), stream));
         tss::launch_fill_int(d.scene_of, 0, CL, stream);
         HIP_CHECK(hipMemsetAsync(d.state, 0, (size_t)U, stream));
         HIP_CHECK(hipMemsetAsync(d.pos, 0, sizeof(float2) * U, stream));

@@ -757,6 +757,26 @@ __global__ __launch_bounds__(256) void k_reduce(DevState d,
 // ---------------------------------------------------------------------------
 // Adaptivity.
 // ---------------------------------------------------------------------------
+// Cell flags (0 or 1 bytes) of cells c0 .. min(c0 + 7, x1) of a row as a
+// mask, bit k for cell c0 + k: three aligned word loads cover any eight
+// bytes (the tables are padded past their last cell), and bits past x1 are
+// cleared.
+__device__ __forceinline__ uint32_t flag_bits8(const uint8_t* __restrict__ f,
+                                               int row, int c0, int x1) {
+    const int a = row + c0;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(f) + (a >> 2);
+    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+    const int sh = (a & 3) * 8;
+    uint64_t bytes = (((uint64_t)w1 << 32) | w0) >> sh;
+    if (sh) bytes |= (uint64_t)w2 << (64 - sh);
+    // byte k (0 or 1) -> bit k: the product gathers the eight low bits in
+    // its top byte
+    const uint32_t m = (uint32_t)(((bytes & 0x0101010101010101ull) *
+                                   0x0102040810204080ull) >> 56);
+    const int n = min(8, x1 - c0 + 1);
+    return m & ((1u << n) - 1u);
+}
+
 // Any surface particle within sqrt-distance r of x (r2 <= t)? Scans the
 // low box and the high box of radius r on the cell-sorted pre-step state.
 // Only cells flagged as holding a surface particle (csurf, set by the
@@ -774,12 +794,8 @@ __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
         bool any = false;
         for (int cy = b.y0; cy <= b.y1 && !any; ++cy) {
             const int row = sc.cbase_l + cy * sc.dlx;
-            for (int c0 = b.x0; c0 <= b.x1 && !any; c0 += 8) {
-                uint32_t f = 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) f |= d.cband[row + min(c0 + k, b.x1)];
-                any = f != 0;
-            }
+            for (int c0 = b.x0; c0 <= b.x1 && !any; c0 += 8)
+                any = flag_bits8(d.cband, row, c0, b.x1) != 0;
         }
         if (!any) return false;
     }
@@ -792,10 +808,7 @@ __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
             const int row = base + cy * dimx;
             for (int c0 = b.x0; c0 <= b.x1; c0 += 8) {
                 // the row's surface-cell flags, eight at a time
-                uint32_t m = 0;
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    m |= (c0 + k <= b.x1 && d.csurf[row + min(c0 + k, b.x1)]) ? (1u << k) : 0u;
+                uint32_t m = flag_bits8(d.csurf, row, c0, b.x1);
                 while (m) {
                     const int c = row + c0 + __builtin_ctz(m);
                     m &= m - 1u;
