@@ -812,9 +812,19 @@ __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
                 while (m) {
                     const int c = row + c0 + __builtin_ctz(m);
                     m &= m - 1u;
+                    // the cell's particles four at a time, their loads in
+                    // flight together (indices clamped into the cell: a
+                    // repeated particle changes no OR)
                     const int en = d.cend[c];
-                    for (int j = d.cstart[c]; j < en; ++j)
-                        if (d.ssurf[j] && dist2(x, d.spos[j]) <= t) return true;
+                    for (int j0 = d.cstart[c]; j0 < en; j0 += 4) {
+                        bool hit = false;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int j = min(j0 + k, en - 1);
+                            hit |= (d.ssurf[j] != 0) & (dist2(x, d.spos[j]) <= t);
+                        }
+                        if (hit) return true;
+                    }
                 }
             }
         }
