@@ -1207,31 +1207,8 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
               keepG = d.seg_base[5];
     const int nseg = seg == 0 ? nl : seg == 1 ? nh : ng;
     const bool events = d.counts->events != 0;
-    for (int q = 0; q < kScanItems; ++q) {
-        const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
-        // without events every live entry is kept and none splits or merges
-        const Flags fq = events ? entry_flags(d, vals, list_g, seg, i, nl, nh, ng)
-                                : Flags{i < nseg ? 1 : 0, 0};
-        const uint64_t m0 = __ballot(fq.p0), m1 = __ballot(fq.p1);
-        if (lane == 0) {
-            wsum[0][wv] = __popcll(m0);
-            wsum[1][wv] = __popcll(m1);
-        }
-        __syncthreads();
-        int pre0 = 0, pre1 = 0, tot0 = 0, tot1 = 0;
-        for (int v = 0; v < kScanWaves; ++v) {
-            const int c0 = wsum[0][v], c1 = wsum[1][v];
-            pre0 += v < wv ? c0 : 0;
-            pre1 += v < wv ? c1 : 0;
-            tot0 += c0;
-            tot1 += c1;
-        }
-        __syncthreads();  // wsum is rewritten by the next chunk
-        // exclusive output positions of this entry
-        const int g0 = run0 + pre0 + __popcll(m0 & lt);
-        const int g1 = run1 + pre1 + __popcll(m1 & lt);
-        run0 += tot0;
-        run1 += tot1;
+    // places entry i at output positions g0 / g1 (flags fq)
+    auto emit = [&](int i, int g0, int g1, const Flags& fq) {
         if (seg == 0 && i < nl) {
             const int u = (int)vals[i];
             const int s = d.sscene[i];
@@ -1285,6 +1262,43 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
             if (fq.p0) nlist_g[g0 - d.seg_base[1]] = p;
             if (fq.p1) nlist_l[keepL + (g1 - d.seg_base[2])] = p;
         }
+    };
+    if (!events) {
+        // every live entry is kept, in order, and none splits or is
+        // re-listed: an entry's position is the tile's base plus its offset
+        // in the tile (the entries before it are live too)
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) {
+            const int off = q * kScanThreads + (int)threadIdx.x;
+            const int i = t * kScanTile + off;
+            emit(i, run0 + off, run1, Flags{i < nseg ? 1 : 0, 0});
+        }
+        return;
+    }
+    for (int q = 0; q < kScanItems; ++q) {
+        const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
+        const Flags fq = entry_flags(d, vals, list_g, seg, i, nl, nh, ng);
+        const uint64_t m0 = __ballot(fq.p0), m1 = __ballot(fq.p1);
+        if (lane == 0) {
+            wsum[0][wv] = __popcll(m0);
+            wsum[1][wv] = __popcll(m1);
+        }
+        __syncthreads();
+        int pre0 = 0, pre1 = 0, tot0 = 0, tot1 = 0;
+        for (int v = 0; v < kScanWaves; ++v) {
+            const int c0 = wsum[0][v], c1 = wsum[1][v];
+            pre0 += v < wv ? c0 : 0;
+            pre1 += v < wv ? c1 : 0;
+            tot0 += c0;
+            tot1 += c1;
+        }
+        __syncthreads();  // wsum is rewritten by the next chunk
+        // exclusive output positions of this entry
+        const int g0 = run0 + pre0 + __popcll(m0 & lt);
+        const int g1 = run1 + pre1 + __popcll(m1 & lt);
+        run0 += tot0;
+        run1 += tot1;
+        emit(i, g0, g1, fq);
     }
 }
 
