@@ -255,6 +255,8 @@ def _batch_for(scs):
 
 
 def test_batch_alone_equals_mix():
+    # the batch holds a band-rule scene, so its reference-rule scenes run the
+    # group pass with a lane per child; alone they run a lane per group
     scs = [scenes.small_two_scale(nx=16, ny=24, mode=MODE_BAND),
            scenes.c2_two_scale(),
            scenes.small_two_scale(nx=20, ny=12, mode=MODE_REFERENCE,
