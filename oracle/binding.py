@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
             "orc_build_grid": (None, [PS, SS, GS]),
             "orc_build_boundary_grid": (None, [PS, SS, GS]),
             "orc_neighbors": (None, [PS, SS, GS, C.c_int, _P, _P]),
+            "orc_pair_sets": (None, [PS, SS, GS, C.c_int, _P, _P]),
             "orc_density_f64": (None, [PS, SS, GS, _P, _P, _P, _P]),
             "orc_density_f32": (None, [PS, SS, GS, _P, _P, _P, _P]),
             "orc_force_f64": (None, [PS, SS, GS, _P, _P, _P, _P]),
@@ -241,6 +242,16 @@ class Scene:
         c = np.zeros((max(n, 1), 5), np.uint32)
         s = np.zeros((max(n, 1), 5), np.uint32)
         lib().orc_neighbors(C.byref(self.p), C.byref(self.s),
+                            C.byref(g["_struct"]), level, ptr(c), ptr(s))
+        return c[:n], s[:n]
+
+    def pair_sets(self, g: dict, level: int) -> tuple[np.ndarray, np.ndarray]:
+        """Per listed target of a level: the pairs with a nonzero state
+        weight, as orc_neighbors' five categories (counts, id checksums)."""
+        n = self.s.n_h if level else self.s.n_l
+        c = np.zeros((max(n, 1), 5), np.uint32)
+        s = np.zeros((max(n, 1), 5), np.uint32)
+        lib().orc_pair_sets(C.byref(self.p), C.byref(self.s),
                             C.byref(g["_struct"]), level, ptr(c), ptr(s))
         return c[:n], s[:n]
 

@@ -226,9 +226,22 @@ static inline uint32_t csum(int id)
     return (uint32_t)(id + 1) * 2654435761u;
 }
 
-void orc_neighbors(const orc_params* p, const orc_state* s,
-                   const orc_grid* g, int level, uint32_t* counts,
-                   uint32_t* sums)
+/* State weights of the pair sums, index 2(s_i & 1) + (s_j & 1) for the
+ * lambda tables: lambda_LH (R:377, 602), lambda_HH (R:414, 681), lambda_HL
+ * (R:451, 808); the deleted weight d(s_j) = {1, 1, 0} on forces from low
+ * particles (R:500, 809). */
+static const int kLamLH[4] = {1, 0, 1, 0};
+static const int kLamHH[4] = {1, 0, 1, 1};
+static const int kLamHL[4] = {1, 1, 1, 0};
+static const int kDel[4] = {1, 1, 0, 0};
+#define LAM(tab, si, sj) (tab[(((si) & 1u) << 1) | ((sj) & 1u)])
+
+/* weighted = 0: geometric sets (every candidate within the cutoff);
+ * weighted = 1: only pairs whose weight in the density or force sum is
+ * nonzero, i.e. the terms those sums add. */
+static void neighbour_sets(const orc_params* p, const orc_state* s,
+                           const orc_grid* g, int level, int weighted,
+                           uint32_t* counts, uint32_t* sums)
 {
     int n = level ? s->n_h : s->n_l;
     const int32_t* list = level ? s->list_h : s->list_l;
@@ -238,6 +251,7 @@ void orc_neighbors(const orc_params* p, const orc_state* s,
     for (int t = 0; t < n; t++) {
         int i = list[t];
         float xi = pos[2 * i], yi = pos[2 * i + 1];
+        const uint8_t si = level ? s->state_h[i] : s->state_l[i];
         uint32_t c[5] = {0, 0, 0, 0, 0}, su[5] = {0, 0, 0, 0, 0};
         /* same-level box, cross-level box, boundary (low) box */
         float rs = level ? H : h;
@@ -253,8 +267,16 @@ void orc_neighbors(const orc_params* p, const orc_state* s,
                 for (int k = st[cc]; k < en[cc]; k++) {
                     int j = so[k];
                     float r = dist_f(xi, yi, pos[2 * j], pos[2 * j + 1]);
-                    if (r <= rd) { c[0]++; su[0] += csum(j); }
-                    if (r != 0.0f && r < rf) { c[2]++; su[2] += csum(j); }
+                    int wd = 1, wf = 1;
+                    if (weighted) {
+                        uint8_t sj = level ? s->state_h[j] : s->state_l[j];
+                        /* LL: density every state (R:335-363), force d(s_j);
+                         * HH: lambda_HH both */
+                        wd = level ? LAM(kLamHH, si, sj) : 1;
+                        wf = level ? LAM(kLamHH, si, sj) : kDel[sj & 3u];
+                    }
+                    if (wd && r <= rd) { c[0]++; su[0] += csum(j); }
+                    if (wf && r != 0.0f && r < rf) { c[2]++; su[2] += csum(j); }
                 }
             }
         }
@@ -274,8 +296,17 @@ void orc_neighbors(const orc_params* p, const orc_state* s,
                 for (int k = st2[cc]; k < en2[cc]; k++) {
                     int j = so2[k];
                     float r = dist_f(xi, yi, pos2[2 * j], pos2[2 * j + 1]);
-                    if (r <= rd2) { c[1]++; su[1] += csum(j); }
-                    if (r != 0.0f && r < rf2) { c[3]++; su[3] += csum(j); }
+                    int wd = 1, wf = 1;
+                    if (weighted) {
+                        uint8_t sj = ol ? s->state_h[j] : s->state_l[j];
+                        /* LH: lambda_LH both; HL: lambda_HL, times d(s_j)
+                         * on the force */
+                        wd = level ? LAM(kLamHL, si, sj) : LAM(kLamLH, si, sj);
+                        wf = level ? LAM(kLamHL, si, sj) * kDel[sj & 3u]
+                                   : LAM(kLamLH, si, sj);
+                    }
+                    if (wd && r <= rd2) { c[1]++; su[1] += csum(j); }
+                    if (wf && r != 0.0f && r < rf2) { c[3]++; su[3] += csum(j); }
                 }
             }
         }
@@ -295,6 +326,20 @@ void orc_neighbors(const orc_params* p, const orc_state* s,
             sums[5 * t + q] = su[q];
         }
     }
+}
+
+void orc_neighbors(const orc_params* p, const orc_state* s,
+                   const orc_grid* g, int level, uint32_t* counts,
+                   uint32_t* sums)
+{
+    neighbour_sets(p, s, g, level, 0, counts, sums);
+}
+
+void orc_pair_sets(const orc_params* p, const orc_state* s,
+                   const orc_grid* g, int level, uint32_t* counts,
+                   uint32_t* sums)
+{
+    neighbour_sets(p, s, g, level, 1, counts, sums);
 }
 
 /* ------------------------------------------------------------------------ */

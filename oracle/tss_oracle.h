@@ -138,6 +138,13 @@ void orc_build_boundary_grid(const orc_params* p, const orc_state* s,
 void orc_neighbors(const orc_params* p, const orc_state* s,
                    const orc_grid* g, int level, uint32_t* counts,
                    uint32_t* sums);
+/* The same five categories restricted to the pairs whose state weight is
+ * nonzero -- the terms the density and force sums add: lambda_LH on low <->
+ * high pairs (R:377, 602), lambda_HH (R:414, 681), lambda_HL (R:451, 808)
+ * and the deleted weight d(s_j) on forces from low particles (R:500, 809). */
+void orc_pair_sets(const orc_params* p, const orc_state* s,
+                   const orc_grid* g, int level, uint32_t* counts,
+                   uint32_t* sums);
 
 /* ---- physics ----------------------------------------------------------- */
 /* Density + Tait pressure for every listed particle, by id. */

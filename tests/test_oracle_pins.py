@@ -258,6 +258,238 @@ def test_fp32_transcription_agrees_with_fp64():
     assert np.max(np.abs(out32[2][lh] / rh[lh] - 1)) < 1e-5
 
 
+# ------------------------------------------- two-level sums, brute force --
+# The state weights as R/WCSPHSolver.cu states them, index 2(s_i&1)+(s_j&1):
+# lambda_LH (R:377), lambda_HH (R:414), lambda_HL (R:451); d(s) on forces
+# from low particles (R:500).
+LAM_LH = (1, 0, 1, 0)
+LAM_HH = (1, 0, 1, 1)
+LAM_HL = (1, 1, 1, 0)
+DEL = (1, 1, 0)
+
+
+def _m4(r, ell):
+    """2-D cubic spline of support ell (R:116-160), numpy FP64: sigma =
+    10/(7 pi ell^2), q = 2r/ell."""
+    q = 2.0 * np.asarray(r, np.float64) / ell
+    sig = 10.0 / (7.0 * math.pi * ell * ell)
+    a, b = np.maximum(2.0 - q, 0.0), np.maximum(1.0 - q, 0.0)
+    return sig * (a ** 3 - 4.0 * b ** 3)
+
+
+def _mixed_state(seed=11, nx=14, ny=14):
+    """An oracle state holding every (s_i, s_j) combination of both levels:
+    low particles active (0), transient parents (1, their children
+    transient) and deleted parents (2, unlisted, their children promoted)."""
+    from twoscalesplit_amd import MODE_BAND, scenes
+    sc = scenes.small_two_scale(nx=nx, ny=ny, nxd=48, nyd=48, mode=MODE_BAND)
+    cfg = sc.cfg
+    o = ob.Scene(ob.params_from_config(cfg), sc.cap_low,
+                 np.zeros((0, 2), np.float32))
+    rng = np.random.default_rng(seed)
+    n, dx = sc.n_low, sc.meta["dx"]
+    P = (sc.low_xy + np.float32(12 * dx)).astype(np.float32)
+    st = rng.choice([0, 1, 2], n, p=[0.4, 0.3, 0.3]).astype(np.uint8)
+    o.a["pos_l"][:n] = P
+    o.a["state_l"][:n] = st
+    ll = np.flatnonzero(st != 2).astype(np.int32)
+    o.a["list_l"][:len(ll)] = ll
+    o.s.n_l = len(ll)
+    par = np.flatnonzero(st != 0)
+    kids = (4 * par[:, None] + np.arange(4)[None, :]).ravel().astype(np.int32)
+    off = np.array([[1, 1], [-1, 1], [1, -1], [-1, -1]], np.float32) * np.float32(0.25 * dx)
+    o.a["pos_h"][kids] = (np.repeat(P[par], 4, axis=0) + np.tile(off, (len(par), 1)) +
+                          rng.uniform(-0.05 * dx, 0.05 * dx, (len(kids), 2))).astype(np.float32)
+    o.a["state_h"][kids] = np.repeat(np.where(st[par] == 1, 1, 0), 4).astype(np.uint8)
+    o.a["list_h"][:len(kids)] = kids
+    o.s.n_h = len(kids)
+    o.a["vel_l"][:n] = rng.normal(0, 0.3, (n, 2)).astype(np.float32)
+    o.a["vel_h"][kids] = rng.normal(0, 0.3, (len(kids), 2)).astype(np.float32)
+    return cfg, o
+
+
+def _dist32_all(xi, Q):
+    """The method's FP32 distance predicate sqrt(fl(fl(dx^2) + fl(dy^2)))
+    from one point to many (R:325-333), and the FP64 distance of the same
+    float positions."""
+    d = (np.float32(xi[0]) - Q[:, 0]).astype(np.float32), (np.float32(xi[1]) - Q[:, 1]).astype(np.float32)
+    r32 = np.sqrt((d[0] * d[0]).astype(np.float32) + (d[1] * d[1]).astype(np.float32)).astype(np.float32)
+    e = np.float64(xi[0]) - Q[:, 0].astype(np.float64), np.float64(xi[1]) - Q[:, 1].astype(np.float64)
+    return r32, np.hypot(e[0], e[1])
+
+
+def _brute_density(cfg, o, hl_cutoff=None):
+    """Both levels' densities by O(N^2) sums over every listed particle
+    (no grid), in FP64 (Appendix A of the method: rho_i = m_L sum_L W_bar +
+    m_H sum_H lambda_LH W_bar over r <= h; rho_a = m_H sum_H lambda_HH W_bar
+    + m_L sum_L lambda_HL W_bar over r <= H)."""
+    h, H = np.float32(cfg.effective_radius), np.float32(cfg.effective_radius_high)
+    hd, Hd = float(h), float(H)
+    hl = H if hl_cutoff is None else np.float32(hl_cutoff)
+    wbar = lambda r: 0.5 * (_m4(r, hd) + _m4(r, Hd))
+    nl, nh, _ = o.n
+    ll, lh = o.a["list_l"][:nl], o.a["list_h"][:nh]
+    PL, PH = o.a["pos_l"], o.a["pos_h"]
+    SL, SH = o.a["state_l"], o.a["state_h"]
+    lam = lambda tab, si, sj: np.asarray(tab, np.float64)[2 * (si & 1) + (sj & 1)]
+    rho_l, rho_h = {}, {}
+    used = {("LH", a, b): 0 for a in (0, 1) for b in (0, 1)}
+    used.update({("HH", a, b): 0 for a in (0, 1) for b in (0, 1)})
+    used.update({("HL", a, b): 0 for a in (0, 1) for b in (0, 1)})
+    for i in ll:
+        r32, r = _dist32_all(PL[i], PL[ll])
+        s1 = wbar(r[r32 <= h]).sum()
+        r32, r = _dist32_all(PL[i], PH[lh])
+        m = r32 <= h
+        w = lam(LAM_LH, int(SL[i]), SH[lh][m]) * wbar(r[m])
+        for sj in (0, 1):
+            used[("LH", int(SL[i]) & 1, sj)] += int(np.sum(m & ((SH[lh] & 1) == sj)))
+        rho_l[i] = cfg.mass_low * s1 + cfg.mass_high * w.sum()
+    for a in lh:
+        r32, r = _dist32_all(PH[a], PH[lh])
+        m = r32 <= H
+        s1 = (lam(LAM_HH, int(SH[a]), SH[lh][m]) * wbar(r[m])).sum()
+        for sj in (0, 1):
+            used[("HH", int(SH[a]) & 1, sj)] += int(np.sum(m & ((SH[lh] & 1) == sj)))
+        r32, r = _dist32_all(PH[a], PL[ll])
+        m = r32 <= hl
+        s2 = (lam(LAM_HL, int(SH[a]), SL[ll][m]) * wbar(r[m])).sum()
+        for sj in (0, 1):
+            used[("HL", int(SH[a]) & 1, sj)] += int(np.sum(m & ((SL[ll] & 1) == sj)))
+        rho_h[a] = cfg.mass_high * s1 + cfg.mass_low * s2
+    return rho_l, rho_h, used
+
+
+def test_m4_numpy_matches_oracle_kernel():
+    for ell in (0.025, 0.0125):
+        for r in np.linspace(0, 1.1, 23) * ell:
+            assert _m4(r, ell) == pytest.approx(L.orc_kernel_w(r, ell), rel=1e-14, abs=1e-300)
+
+
+def test_two_level_density_brute_force():
+    """orc_density_f64 on two levels against an O(N^2) sum over every listed
+    pair with the state weight tables, on a state holding every (s_i, s_j)
+    pair: cell scan boxes, lambda tables, the low<-high cutoff r <= h and the
+    high<-low cutoff r <= H (R:468) all enter; 1e-12 relative."""
+    cfg, o = _mixed_state()
+    g = o.grid()
+    rl, pl, rh, ph = o.density64(g)
+    bl, bh, used = _brute_density(cfg, o)
+    # every table entry is exercised by at least a few pairs
+    assert all(v > 0 for v in used.values()), used
+    for i, v in bl.items():
+        assert rl[i] == pytest.approx(v, rel=1e-12), ("low", i)
+    for a, v in bh.items():
+        assert rh[a] == pytest.approx(v, rel=1e-12), ("high", a)
+    # Tait of the same density (R:1421-1423)
+    B, r0 = cfg.tait_coefficient, cfg.rest_density
+    for i in list(bl)[:50]:
+        assert pl[i] == pytest.approx(B * ((rl[i] / r0) ** 7 - 1), rel=1e-12, abs=1e-9 * B)
+    # the check discriminates: the high<-low cutoff at h instead of H moves
+    # high densities by far more than the tolerance
+    _, bh_wrong, _ = _brute_density(cfg, o, hl_cutoff=cfg.effective_radius)
+    dev = max(abs(bh_wrong[a] / bh[a] - 1) for a in bh)
+    assert dev > 1e-3, dev
+
+
+def test_pair_sets_equal_weighted_brute_force():
+    """orc_pair_sets (the pairs the density and force sums add) against brute
+    force over every listed particle with the weight tables."""
+    cfg, o = _mixed_state(seed=5, nx=10, ny=10)
+    g = o.grid()
+    h, H = np.float32(cfg.effective_radius), np.float32(cfg.effective_radius_high)
+    nl, nh, _ = o.n
+    ll, lh = o.a["list_l"][:nl], o.a["list_h"][:nh]
+    PL, PH, SL, SH = o.a["pos_l"], o.a["pos_h"], o.a["state_l"], o.a["state_h"]
+    csum = lambda ids: int(np.sum((np.asarray(ids, np.uint64) + 1) * 2654435761) % (1 << 32))
+    lam = lambda tab, si, sj: np.asarray(tab)[2 * (si & 1) + (sj & 1)] != 0
+    dele = lambda sj: np.asarray(DEL)[sj] != 0
+    for level in (0, 1):
+        counts, sums = o.pair_sets(g, level)
+        lst = lh if level else ll
+        for t, i in enumerate(lst):
+            P, S = (PH, SH) if level else (PL, SL)
+            si = int(S[i])
+            rs, _ = _dist32_all(P[i], PH[lh] if level else PL[ll])
+            rc, _ = _dist32_all(P[i], PL[ll] if level else PH[lh])
+            if level:
+                ws = lam(LAM_HH, si, SH[lh])
+                wcd = lam(LAM_HL, si, SL[ll])
+                wcf = wcd & dele(SL[ll])
+                same, cross, rd, rdx = lh, ll, H, H
+                wsd = wsf = ws
+            else:
+                wsd = np.ones(len(ll), bool)
+                wsf = dele(SL[ll])
+                wcd = wcf = lam(LAM_LH, si, SH[lh])
+                same, cross, rd, rdx = ll, lh, h, h
+            sets = [same[wsd & (rs <= rd)], cross[wcd & (rc <= rdx)],
+                    same[wsf & (rs != 0) & (rs < rd)], cross[wcf & (rc != 0) & (rc < h)]]
+            for q in range(4):
+                assert counts[t, q] == len(sets[q]), (level, i, q)
+                assert sums[t, q] == csum(sets[q]), (level, i, q)
+        # the weights drop pairs that the geometric sets hold
+        gc, _ = o.neighbors(g, level)
+        assert np.all(counts <= gc) and np.any(counts[:, :4] < gc[:, :4])
+
+
+def test_promoted_lattice_bulk_density():
+    """Promoted children (state 0) with their parents deleted: the high level
+    alone, whose blended kernel is cut at r <= H, reads rho0 (1 + F)/2 in
+    bulk, F = int_{r<=H} W_h dA (quadrature); active low particles beside
+    them read rho0 from their own level (lambda_LH[0,0] adds the children
+    only where they lie within h)."""
+    from twoscalesplit_amd import MODE_BAND, scenes
+    sc = scenes.small_two_scale(nx=24, ny=24, nxd=64, nyd=64, mode=MODE_BAND)
+    cfg = sc.cfg
+    dx = sc.meta["dx"]
+    h, H = cfg.effective_radius, cfg.effective_radius_high
+    F, _ = integrate.quad(lambda r: L.orc_kernel_w(r, h) * 2 * math.pi * r, 0, H,
+                          epsabs=1e-14, epsrel=1e-13)
+    o = ob.Scene(ob.params_from_config(cfg), sc.cap_low, np.zeros((0, 2), np.float32))
+    n = sc.n_low
+    # an unjittered lattice: every reading then lies between the two ideal
+    # lattice sums below
+    I, J = np.meshgrid(np.arange(24), np.arange(24))
+    P = (np.stack([I.ravel(), J.ravel()], 1) * dx + (16.5 * dx)).astype(np.float32)
+    o.a["pos_l"][:n] = P
+    o.a["state_l"][:n] = 2          # every parent deleted (promoted groups)
+    o.s.n_l = 0
+    kids = np.arange(4 * n, dtype=np.int32)
+    off = np.array([[1, 1], [-1, 1], [1, -1], [-1, -1]], np.float32) * np.float32(0.25 * dx)
+    o.a["pos_h"][kids] = (np.repeat(P, 4, axis=0) + np.tile(off, (n, 1))).astype(np.float32)
+    o.a["state_h"][kids] = 0
+    o.a["list_h"][:4 * n] = kids
+    o.s.n_h = 4 * n
+    g = o.grid()
+    _, _, rh, _ = o.density64(g)
+    c = P.mean(axis=0)
+    Q = o.a["pos_h"][kids]
+    inner = np.all(np.abs(Q - c) < 0.3 * (P.max(axis=0) - P.min(axis=0)), axis=1)
+    ratio = rh[kids][inner] / cfg.rest_density
+    assert inner.sum() > 100
+    # the ideal lattice of spacing H/4 reads (1 + F)/2 to 1 % when the four
+    # sites at exactly r = H count (quadrature), 1.4 % less without them:
+    # the blended kernel jumps there (W_h(H) = sigma_h), and FP32 rounding
+    # of the distance puts those sites on either side of the cutoff
+    s = 0.25 * float(H)
+    k = np.arange(-5, 6)
+    X, Y = np.meshgrid(k * s, k * s)
+    r = np.hypot(X, Y).ravel()
+    wbar = 0.5 * (_m4(r, float(h)) + _m4(r, float(H)))
+    mass = cfg.mass_high / cfg.rest_density
+    est_in = mass * wbar[r <= float(H) * (1 + 1e-9)].sum()
+    est_out = mass * wbar[r < float(H) * (1 - 1e-9)].sum()
+    assert est_in == pytest.approx((1 + F) / 2, rel=0.01)
+    assert est_out < est_in * 0.99
+    assert np.all(ratio >= est_out * (1 - 1e-5)) and np.all(ratio <= est_in * (1 + 1e-5)), (
+        ratio.min(), ratio.max(), est_out, est_in)
+    # more sharply: each reading is the ideal sum without the ring plus an
+    # integer number (0..4) of ring sites of weight m W_bar(H)
+    k = (ratio - est_out) / ((est_in - est_out) / 4)
+    assert np.all(np.abs(k - np.round(k)) < 1e-3), k
+
+
 # -------------------------------------------------------------- adaptivity --
 def test_split_geometry_and_clamp():
     from twoscalesplit_amd import MODE_REFERENCE, scenes
