@@ -81,7 +81,8 @@ typedef struct tss_config {
     float merge_radius;       /* merge when all children are farther       */
     int promote_min_age, promote_max_age;
     float promote_fluc;
-    int cap_high;             /* max live high particles (<= 4*cap_low)     */
+    int cap_high;             /* max live high particles (<= 4*cap_low);
+                                 0 = no cap (all 4*cap_low child slots)    */
     int dt_mode;              /* TSS_DT_*                                  */
     float cfl, dt_max;
     /* pressure-only relaxation of the children injected by the previous
@@ -151,7 +152,7 @@ typedef struct tss_counts_t {
  * WCSPHConfig fields, derives the grid dimensions (ceil of FP32 quotient)
  * and sets the remaining fields to defaults (gravity 9.81, split_x 0.5,
  * surface_ratio 0.9, surface_ratio_high 0.9 kappa(H/h), band/merge radius
- * h, plain mode, fixed dt, cap 0).
+ * h, plain mode, fixed dt, cap_high 0 = no cap).
  * TSS_E_INVALID if xs >= xe or ys >= ye, like the reference's exception. */
 int tss_config_init(tss_config* cfg, float xs, float ys, float xe, float ye,
                     float effective_radius, float effective_radius_high,
@@ -183,7 +184,13 @@ int tss_counts(tss_solver* s, tss_counts_t* out);
 int tss_read(tss_solver* s, int field, void* dst, size_t bytes);
 int tss_write(tss_solver* s, int field, const void* src, size_t bytes);
 /* Sets the list counts (n_low, n_high, n_groups) after tss_write of lists:
- * re-seeding a state for tests and restart. */
+ * re-seeding a state for tests and restart. Lists, counts and states written
+ * from the host are checked before the next call that launches on them
+ * (tss_step, tss_step_graph, tss_neighbors, tss_pair_sets,
+ * tss_work_counts): ids in range without repeats, no deleted particle
+ * listed, no active group parent, no child of an unlisted group (the
+ * invariants that bound list growth); otherwise that call returns
+ * TSS_E_INVALID and launches nothing. */
 int tss_set_counts(tss_solver* s, int n_low, int n_high, int n_groups);
 /* Device pointer of a field (valid until tss_destroy), replacing GL
  * interop for device-side consumers. */
@@ -197,6 +204,18 @@ int tss_conservation(tss_solver* s, double out[5], double err_bound[5]);
  * (see oracle's orc_neighbors for the categories). Builds the grid of the
  * current lists; does not advance. counts/sums hold 5*n entries. */
 int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums,
+                  size_t n);
+/* The pairs the step's own density and force kernels add, in the categories
+ * of tss_neighbors restricted to nonzero state weights: [0] same-level
+ * density, [1] cross-level density, [2] same-level force, [3] cross-level
+ * force, [4] wall candidates of the force. The kernels run on the current
+ * state in a recording variant -- the same candidate masks, scans and row
+ * and pass skips as a step -- after a grid build of the current lists; no
+ * state is advanced (relaxation sub-steps are not run). These are the sets
+ * of updateDensityCell* (R/WCSPHSolver.cu:335-474) and updateAccCell*
+ * (R/WCSPHSolver.cu:476-885) with their lambda tables; no counterpart
+ * entry point in the reference. */
+int tss_pair_sets(tss_solver* s, int level, uint32_t* counts, uint32_t* sums,
                   size_t n);
 /* Device time per phase (ms), averaged over every step taken (eager or
  * graph-replayed) since timing was switched on, from hipEvents recorded on

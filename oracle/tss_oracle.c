@@ -492,7 +492,9 @@ void orc_adapt(const orc_params* p, orc_state* s, const orc_grid* g,
                     !near_wall(p, g, s, pph[2 * a], pph[2 * a + 1]);
     }
 
-    /* 1. split candidates in sorted order, admitted against the cap */
+    /* 1. split candidates in sorted order, admitted against the cap
+     * (cap_h 0: no cap, every child slot 4 cap_l) */
+    const long long cap_h = p->cap_h > 0 ? (long long)p->cap_h : 4LL * s->cap_l;
     for (int k = 0; k < s->n_l; k++) {
         int i = g->sorted_l[k];
         if (s->state_l[i] != ORC_ACTIVE) {
@@ -505,8 +507,7 @@ void orc_adapt(const orc_params* p, orc_state* s, const orc_grid* g,
             cand = in_band(p, g, s, ppl, pph, surf_l, surf_h, ppl[2 * i],
                            ppl[2 * i + 1], p->r_band);
         }
-        if (cand &&
-            4LL * (s->n_g + out->n_split + 1) <= (long long)p->cap_h) {
+        if (cand && 4LL * (s->n_g + out->n_split + 1) <= cap_h) {
             out->split_ids[out->n_split++] = i;
         }
     }

@@ -4,10 +4,18 @@ The oracle is re-seeded from the GPU state before the step; every phase is
 then checked on the same inputs the GPU used:
 
 * grid (hashes, stable sort order, cell ranges, boundary grid): bit-exact
+* the pairs the step's own density and force kernels add (tss_pair_sets: the
+  kernels' recording variant, same masks, scans and skips), per target and
+  category, counts and id checksums: bit-exact against orc_pair_sets
 * density (FP32, compensated) vs the FP64 oracle:
       |rho - rho64| / rho64                       <= TOL_RHO
       |p - p64| / max(|p64|, B)                   <= TOL_P
-  (p/B + 1 = (rho/rho0)^7: the pressure is checked through the Tait ratio)
+  (p/B + 1 = (rho/rho0)^7: the pressure is checked through the Tait ratio;
+  dp/drho = 7 B (rho/rho0)^6 / rho0, so an FP32 density -- half an ulp,
+  6e-8 -- moves p by ~7 B 6e-8 = 0.09 Pa however small |p| is. Reported
+  beside it, not asserted: |p - p64| / max(|p64|, 7 B 2^-23), p50/p99/max)
+* visual quantity (setQuantities, R/WCSPHSolver.cu:1030-1051):
+  min(|rho - rho0|, 400) / 400 of the step's FP32 density, bit-exact
 * force vs the FP64 oracle fed the GPU's rho and p:
       |a - a64| / max(|a64|, S)                   <= TOL_ACC
   S = sum of the magnitudes of the pair, wall and gravity terms (the
@@ -34,10 +42,28 @@ def _listed(snap, level):
     return snap["LIST_H"][:c["n_high"]]
 
 
-def check_step(sol, cfg, bnd_xy, report=None):
-    """Advance `sol` one step and check it against the oracle."""
+def check_pairs(sol, o, g) -> tuple[int, int]:
+    """The hot-path pair sets of the current state against the oracle's
+    (bit-exact); returns the number of listed targets of each level."""
+    n = []
+    for level in (0, 1):
+        c_gpu, s_gpu = sol.pair_sets(level)
+        c_orc, s_orc = o.pair_sets(g, level)
+        np.testing.assert_array_equal(c_gpu, c_orc, err_msg=f"pair counts, level {level}")
+        np.testing.assert_array_equal(s_gpu, s_orc, err_msg=f"pair checksums, level {level}")
+        n.append(len(c_gpu))
+    return n[0], n[1]
+
+
+def check_step(sol, cfg, bnd_xy, report=None, pairs=True):
+    """Advance `sol` one step and check it against the oracle (with pairs:
+    first the kernels' pair sets on the pre-step state)."""
     pre = sol.snapshot()
     cap = pre["counts"]["cap_low"]
+    if pairs and getattr(cfg, "relax_iters", 0) == 0:
+        o0 = ob.Scene(ob.params_from_config(cfg), cap, bnd_xy)
+        o0.load_snapshot(pre)
+        check_pairs(sol, o0, o0.grid())
     sol.step(1)
     post = sol.snapshot()
     cnt = sol.counts()
@@ -77,8 +103,21 @@ def check_step(sol, cfg, bnd_xy, report=None):
         ep = np.abs(pg - p64[ids]) / np.maximum(np.abs(p64[ids]), B)
         err[f"rho{lvl}"] = float(er.max())
         err[f"p{lvl}"] = float(ep.max())
+        epr = np.abs(pg - p64[ids]) / np.maximum(np.abs(p64[ids]), 7.0 * B * 2.0 ** -23)
+        err[f"p_rel{lvl}"] = [float(np.percentile(epr, 50)),
+                              float(np.percentile(epr, 99)), float(epr.max())]
         assert er.max() <= TOL_RHO, (lvl, er.max(), ids[er.argmax()])
         assert ep.max() <= TOL_P, (lvl, ep.max(), ids[ep.argmax()])
+
+    # ---- visual quantity: bit-exact from the step's density -------------
+    r0 = np.float32(cfg.rest_density)
+    for ids, R, V in ((il, "RHO_L", "VIS_L"), (ih, "RHO_H", "VIS_H")):
+        if len(ids) == 0:
+            continue
+        rho = post[R][ids].astype(np.float32)
+        want = np.minimum(np.abs(rho - r0), np.float32(400.0)) / np.float32(400.0)
+        np.testing.assert_array_equal(post[V][ids], want.astype(np.float32),
+                                      err_msg=V)
 
     # ---- force (on the GPU's rho/p) --------------------------------------
     o.a["rho_l"][:] = post["RHO_L"]

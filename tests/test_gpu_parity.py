@@ -14,7 +14,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from parity import check_step
+from parity import check_pairs, check_step
 from twoscalesplit_amd import MODE_BAND, MODE_REFERENCE, Batch, Solver
 from twoscalesplit_amd import scenes
 
@@ -184,6 +184,9 @@ def test_neighbour_sets_bit_exact(oracle):
             assert len(c_gpu) > 0
             np.testing.assert_array_equal(c_gpu, c_orc)
             np.testing.assert_array_equal(s_gpu, s_orc)
+        # the sets the hot-path kernels themselves add (state weights on)
+        nl, nh = check_pairs(sol, o, g)
+        assert nl > 0 and nh > 0
     finally:
         sol.close()
 

@@ -189,6 +189,7 @@ PROTOTYPES = {
     "tss_synchronize": (_I, [_P]),
     "tss_conservation": (_I, [_P, _P, _P]),
     "tss_neighbors": (_I, [_P, _I, _P, _P, _SZ]),
+    "tss_pair_sets": (_I, [_P, _I, _P, _P, _SZ]),
     "tss_enable_timing": (_I, [_P, _I]),
     "tss_phase_times": (_I, [_P, _P]),
     "tss_work_counts": (_I, [_P, _P]),

@@ -188,6 +188,12 @@ struct DevState {
     uint8_t* cband;
     float4* rtmp;             // [4 CL] relaxation scratch (x', v') per target;
                               // allocated when a scene enables relaxation
+    // [5 CL * 10] pair-recording output of the density and force kernels'
+    // recording variant (null in the step): per unified id, the counts of
+    // the pairs each sum added (same-level density, cross-level density,
+    // same-level force, cross-level force, wall candidates) and their id
+    // checksums
+    uint32_t* pairrec;
 };
 
 // Launch shape of one neighbour kernel: block size (128, 256, 512), LDS
@@ -242,6 +248,11 @@ void launch_neighbors_n(const DevState& d, const StepShape& sh, int scene,
 void launch_grid_only(const DevState& d, const StepShape& sh, void* stream);
 void launch_work_counts(const DevState& d, const StepShape& sh,
                         unsigned long long* out, void* stream);
+// Grid build of the current lists, then the density and force kernels in
+// their recording variant: no state is advanced; d.pairrec receives the
+// pairs each sum added (kPairRecWords words per unified id).
+constexpr int kPairRecWords = 10;
+void launch_pair_record(const DevState& d, const StepShape& sh, void* stream);
 void launch_fill_int(int* p, int value, long long n, void* stream);
 void launch_iota(int* p, int first, int n, void* stream);
 // Frame export sections (any pointer may be null: field not exported).

@@ -461,7 +461,7 @@ struct tss_batch {
         sh.cap_binds = 0;
         sh.band_any = 0;
         for (const SceneHost& s : scenes) {
-            if (s.alive && (long long)s.cfg.cap_high < 4LL * s.lcap) sh.cap_binds = 1;
+            if (s.alive && effective_cap_high(s.cfg, s.lcap) < 4LL * s.lcap) sh.cap_binds = 1;
             if (s.alive && s.cfg.mode == 2) sh.band_any = 1;
         }
         long long maxkey = 1;
@@ -480,7 +480,15 @@ struct tss_batch {
         invalidate_graphs();
     }
 
+    // Executable graphs are destroyed only once the stream has drained: a
+    // replay still in flight keeps reading its exec's kernel arguments, and
+    // HIP does not document a deferred destruction (update_shape is reached
+    // from set_tuning and write paths right after enqueued replays).
     void invalidate_graphs() {
+        bool any = false;
+        for (auto& g : graph) any |= g != nullptr;
+        for (auto& t : tgraph) any |= t.exec != nullptr;
+        if (any && stream) (void)hipStreamSynchronize(stream);
         for (auto& g : graph) {
             if (g) {
                 (void)hipGraphExecDestroy(g);
@@ -575,7 +583,7 @@ struct tss_batch {
         s.pmin = c.promote_min_age;
         s.pmax = c.promote_max_age;
         s.pfluc = c.promote_fluc;
-        s.cap_h = c.cap_high;
+        s.cap_h = c.cap_high;  // 0 (no cap) is resolved by add_scene
         s.dt_mode = c.dt_mode;
         s.cfl = c.cfl;
         s.dt_max = c.dt_max;
@@ -585,6 +593,11 @@ struct tss_batch {
         s.relax_frac = c.relax_dt_frac;
         s.hh_inner = 2.0 * (double)s.H <= (double)s.h ? 1 : 0;
         return s;
+    }
+
+    // cap_high 0 means no cap: every child slot (4 cap_low) is available
+    static long long effective_cap_high(const tss_config& c, long long cap_low) {
+        return c.cap_high > 0 ? (long long)c.cap_high : 4LL * cap_low;
     }
 
     static void validate(const tss_config& c, int n_low, int cap_low, int n_bnd) {
@@ -671,6 +684,7 @@ struct tss_batch {
         sd.cbase_h = sh_.cbh;
         sd.lbase = sh_.lbase;
         sd.lcap = cap_low;
+        sd.cap_h = effective_cap_high(c, cap_low);
         sd.bbase = sh_.bbase;
         sd.nb = n_bnd;
 
@@ -839,6 +853,7 @@ struct tss_batch {
     }
 
     void step_eager(int n) {
+        check_host_writes();
         for (int i = 0; i < n; ++i) {
             void* ev = timing ? (void*)tev[acquire_slot()] : nullptr;
             tss::launch_step(d, sh, stream, ev, 0u);
@@ -901,6 +916,7 @@ struct tss_batch {
     }
 
     void step_graph(int n) {
+        check_host_writes();
         const int pairs = n / 2;
         if (pairs > 0) {
             const int par = sh.list_parity;
@@ -1201,6 +1217,7 @@ struct tss_batch {
             HIP_CHECK(hipStreamSynchronize(stream));
             mark_host_change();
             invalidate_graphs();
+            if (field == TSS_STATE_L || field == TSS_STATE_H) lists_dirty = true;
             return;
         }
         if (field == TSS_DT) {
@@ -1236,6 +1253,7 @@ struct tss_batch {
             HIP_CHECK(hipStreamSynchronize(stream));
         }
         mark_host_change();
+        lists_dirty = true;
     }
 
     // ---- frame export ------------------------------------------------------
@@ -1483,36 +1501,77 @@ struct tss_batch {
             const uint8_t* st = (const uint8_t*)(in + offs[k]);
             for (size_t i = 0; i < sizes[k]; ++i) require(st[i] <= 2, "unknown particle state in snapshot");
         }
-        // lists: ids in range, no repeats
-        const int* ll = (const int*)(in + L.ll);
-        const int* lh = (const int*)(in + L.lh);
-        const int* lg = (const int*)(in + L.lg);
-        auto check_list = [](const int* l, int n, size_t range, const char* what) {
+        (void)hc;
+        check_lists((const int*)(in + L.ll), hd.n_low, (const int*)(in + L.lh), hd.n_high,
+                    (const int*)(in + L.lg), hd.n_groups, (const uint8_t*)(in + offs[6]), lc,
+                    "in snapshot");
+        return hd;
+    }
+
+    // The list invariants that bound every later step's list growth (and
+    // so keep the device-side compaction within the list buffers): each
+    // list holds ids in range without repeats; listed low particles are
+    // active or transient; group parents are transient or deleted (so they
+    // cannot split again); every listed child belongs to a listed group.
+    // Ids are scene-local; st_l holds the low states (lc of them).
+    static void check_lists(const int* ll, int nl, const int* lh, int nh, const int* lg,
+                            int ng, const uint8_t* st_l, size_t lc, const char* where) {
+        const std::string at = std::string(" ") + where;
+        auto check_list = [&](const int* l, int n, size_t range, const char* what) {
             std::vector<uint8_t> seen(range, 0);
             for (int i = 0; i < n; ++i) {
                 require(l[i] >= 0 && (size_t)l[i] < range && !seen[l[i]],
-                        std::string("invalid ") + what + " list in snapshot");
+                        std::string("invalid ") + what + " list" + at);
                 seen[l[i]] = 1;
             }
         };
-        check_list(ll, hd.n_low, lc, "low");
-        check_list(lh, hd.n_high, hc, "high");
-        check_list(lg, hd.n_groups, lc, "group");
-        // the list invariants that bound every later step's list growth:
-        // listed low particles are active or transient, group parents are
-        // transient or deleted (so they cannot split again), and every
-        // listed child belongs to a listed group
-        const uint8_t* st_l = (const uint8_t*)(in + offs[6]);
+        check_list(ll, nl, lc, "low");
+        check_list(lh, nh, 4 * lc, "high");
+        check_list(lg, ng, lc, "group");
         std::vector<uint8_t> group(lc, 0);
-        for (int i = 0; i < hd.n_groups; ++i) {
-            require(st_l[lg[i]] != kStateActive, "group parent is active in snapshot");
+        for (int i = 0; i < ng; ++i) {
+            require(st_l[lg[i]] != kStateActive, "group parent is active" + at);
             group[lg[i]] = 1;
         }
-        for (int i = 0; i < hd.n_low; ++i)
-            require(st_l[ll[i]] != kStateDeleted, "deleted particle listed in snapshot");
-        for (int i = 0; i < hd.n_high; ++i)
-            require(group[lh[i] >> 2], "child of an unlisted group in snapshot");
-        return hd;
+        for (int i = 0; i < nl; ++i)
+            require(st_l[ll[i]] != kStateDeleted, "deleted particle listed" + at);
+        for (int i = 0; i < nh; ++i)
+            require(group[lh[i] >> 2], "child of an unlisted group" + at);
+    }
+
+    // Host writes of lists, counts or states (tss_write, tss_set_counts) may
+    // leave lists that break the invariants above; they are checked before
+    // the next launch that reads the lists, on the device's current lists,
+    // counts and states. A failing state is refused (TSS_E_INVALID) and
+    // nothing is launched until the caller writes a valid one.
+    bool lists_dirty = false;
+    void check_host_writes() {
+        if (!lists_dirty) return;
+        const tss::DevCounts c = read_counts();
+        const int par = sh.list_parity;
+        for (int sid = 0; sid < max_scenes; ++sid) {
+            const SceneHost& s = scenes[sid];
+            if (!s.alive) continue;
+            const tss::SceneCount sc = read_scnt(sid);
+            require(sc.n_l == c.n_l && sc.n_h == c.n_h && sc.n_g == c.n_g,
+                    "host writes need a batch of one scene");
+            std::vector<int> ll(c.n_l), lh(c.n_h), lg(c.n_g);
+            std::vector<uint8_t> st((size_t)s.lcap);
+            if (c.n_l)
+                HIP_CHECK(hipMemcpyAsync(ll.data(), d.list_l[par], 4 * (size_t)c.n_l, hipMemcpyDeviceToHost, stream));
+            if (c.n_h)
+                HIP_CHECK(hipMemcpyAsync(lh.data(), d.list_h[par], 4 * (size_t)c.n_h, hipMemcpyDeviceToHost, stream));
+            if (c.n_g)
+                HIP_CHECK(hipMemcpyAsync(lg.data(), d.list_g[par], 4 * (size_t)c.n_g, hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(st.data(), d.state + s.lbase, (size_t)s.lcap, hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            for (int& x : ll) x -= s.lbase;
+            for (int& x : lh) x -= d.CL + 4 * s.lbase;
+            for (int& x : lg) x -= s.lbase;
+            check_lists(ll.data(), c.n_l, lh.data(), c.n_h, lg.data(), c.n_g, st.data(),
+                        (size_t)s.lcap, "after host writes");
+        }
+        lists_dirty = false;
     }
 
     int restore(const void* src, size_t bytes) {
@@ -1599,6 +1658,7 @@ struct tss_batch {
         sc.n_g = ng;
         HIP_CHECK(hipMemcpyAsync(d.scnt + sid, &sc, sizeof(sc), hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
+        lists_dirty = true;
     }
 
     void counts(int sid, tss_counts_t* o) {
@@ -1637,6 +1697,7 @@ struct tss_batch {
         int alive = 0;
         for (auto& x : scenes) alive += x.alive;
         require(alive == 1 && scenes[sid].alive, "neighbour inspection needs a batch of one scene");
+        check_host_writes();
         tss::DevCounts c = read_counts();
         const int nl = level ? c.n_h : c.n_l;
         require(n == (size_t)nl, "n must equal the level's list length (" + std::to_string(nl) + ")");
@@ -1657,10 +1718,53 @@ struct tss_batch {
         HIP_CHECK(hipGetLastError());
     }
 
+    // The step's density and force kernels in their recording variant on
+    // the current state: per listed particle of a level (list order), the
+    // pairs each sum adds, as counts and id checksums in the neighbour
+    // categories.
+    void pair_sets(int sid, int level, uint32_t* counts_out, uint32_t* sums_out, size_t n) {
+        int alive = 0;
+        for (auto& x : scenes) alive += x.alive;
+        require(alive == 1 && scenes[sid].alive, "pair recording needs a batch of one scene");
+        check_host_writes();
+        const tss::DevCounts c = read_counts();
+        const int nl = level ? c.n_h : c.n_l;
+        require(n == (size_t)nl, "n must equal the level's list length (" + std::to_string(nl) + ")");
+        const size_t U = 5 * (size_t)d.CL, words = (size_t)tss::kPairRecWords * U;
+        uint32_t* rec = nullptr;
+        HIP_CHECK(hipMalloc(&rec, 4 * words));
+        std::vector<uint32_t> host(words);
+        std::vector<int> list(n);
+        hipError_t e = hipMemsetAsync(rec, 0, 4 * words, stream);
+        if (e == hipSuccess) {
+            tss::DevState d2 = d;
+            d2.pairrec = rec;
+            tss::launch_pair_record(d2, sh, stream);
+            e = hipMemcpyAsync(host.data(), rec, 4 * words, hipMemcpyDeviceToHost, stream);
+        }
+        const int par = sh.list_parity;
+        if (e == hipSuccess && n)
+            e = hipMemcpyAsync(list.data(), level ? d.list_h[par] : d.list_l[par], 4 * n,
+                               hipMemcpyDeviceToHost, stream);
+        const hipError_t e2 = hipStreamSynchronize(stream);
+        (void)hipFree(rec);
+        HIP_CHECK(e);
+        HIP_CHECK(e2);
+        HIP_CHECK(hipGetLastError());
+        for (size_t t = 0; t < n; ++t) {
+            const uint32_t* r = host.data() + (size_t)list[t] * tss::kPairRecWords;
+            for (int q = 0; q < 5; ++q) {
+                counts_out[5 * t + q] = r[q];
+                sums_out[5 * t + q] = r[5 + q];
+            }
+        }
+    }
+
     void work_counts(int sid, unsigned long long* out) {
         int alive = 0;
         for (auto& x : scenes) alive += x.alive;
         require(alive == 1 && scenes[sid].alive, "work counts need a batch of one scene");
+        check_host_writes();
         unsigned long long* dev = nullptr;
         HIP_CHECK(hipMalloc(&dev, tss::kWorkCounts * sizeof(unsigned long long)));
         tss::launch_work_counts(d, sh, dev, stream);
@@ -1882,6 +1986,13 @@ int tss_neighbors(tss_solver* s, int level, uint32_t* counts, uint32_t* sums, si
     return guarded([&] {
         require(s && (level == 0 || level == 1) && ((counts && sums) || n == 0), "invalid argument");
         s->b.neighbors(s->sid, level, counts, sums, n);
+    });
+}
+
+int tss_pair_sets(tss_solver* s, int level, uint32_t* counts, uint32_t* sums, size_t n) {
+    return guarded([&] {
+        require(s && (level == 0 || level == 1) && (n == 0 || (counts && sums)), "invalid argument");
+        s->b.pair_sets(s->sid, level, counts, sums, n);
     });
 }
 

@@ -82,6 +82,18 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x) {
     s = t;
 }
 
+// Order-independent id checksum of neighbour sets: (id + 1) * 2654435761
+// mod 2^32, summed over a set (ids scene-local).
+__device__ __forceinline__ uint32_t csum(int id) {
+    return (uint32_t)(id + 1) * 2654435761u;
+}
+
+// Scene-local id of a unified particle id u (low ids from lbase, children
+// from 4 lbase).
+__device__ __forceinline__ int local_id(const DevState& d, const SceneDev& sc, int u) {
+    return u < d.CL ? u - sc.lbase : (u - d.CL) - 4 * sc.lbase;
+}
+
 struct Box {
     int x0, x1, y0, y1;
 };
@@ -1386,9 +1398,6 @@ __global__ __launch_bounds__(256) void k_finalize(DevState d) {
 // Neighbour-set inspection (tests): same boxes, runs and predicates as the
 // density and force kernels.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t csum(int id) {
-    return (uint32_t)(id + 1) * 2654435761u;
-}
 
 __global__ void k_neighbors(DevState d, const uint32_t* __restrict__ vals,
                             const int* __restrict__ list, int n, int scene,
@@ -1856,6 +1865,21 @@ void launch_work_counts(const DevState& d, const StepShape& sh,
     enqueue_grid(d, sh, st);
     (void)hipMemsetAsync(out, 0, kWorkCounts * sizeof(unsigned long long), st);
     k_work<<<sh.grid_elem, 256, 0, st>>>(d, out);
+}
+
+// Pair recording (tests): the grid of the current lists, then both levels'
+// density and force kernels in their recording variant.
+void launch_pair_record(const DevState& d, const StepShape& sh, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const int cur = enqueue_grid(d, sh, st);
+    const uint32_t* vals = d.vals[cur];
+    const uint32_t* keys = d.keys[cur];
+    const int gl = blocks_for(sh.cap_alive, 256, 16384);
+    const int gh = blocks_for(4 * sh.cap_alive, 256, 16384);
+    k_density<false, 256, 16, true><<<gl, 256, 0, st>>>(d, vals, keys, 0);
+    k_density<true, 256, 16, true><<<gh, 256, 0, st>>>(d, vals, keys, 0);
+    k_force<false, 256, 16, true><<<gl, 256, 0, st>>>(d, vals, 0);
+    k_force<true, 256, 16, true><<<gh, 256, 0, st>>>(d, vals, 0);
 }
 
 // Neighbour inspection of the first n entries of a level's current list.

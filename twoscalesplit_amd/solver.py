@@ -202,6 +202,17 @@ class Solver:
                 "tss_neighbors")
         return c, s
 
+    def pair_sets(self, level: int) -> tuple[np.ndarray, np.ndarray]:
+        """The pairs the step's density and force kernels add, per listed
+        particle of a level (tss_pair_sets; the neighbors() categories)."""
+        cnt = self.counts()
+        n = cnt["n_high"] if level else cnt["n_low"]
+        c = np.zeros((n, 5), np.uint32)
+        s = np.zeros((n, 5), np.uint32)
+        N.check(N.lib().tss_pair_sets(self._h, level, N.ptr(c), N.ptr(s), n),
+                "tss_pair_sets")
+        return c, s
+
     def enable_timing(self, on: bool = True) -> None:
         N.check(N.lib().tss_enable_timing(self._h, 1 if on else 0),
                 "tss_enable_timing")
@@ -251,9 +262,9 @@ class Solver:
     def snapshot(self) -> dict:
         """Every by-id field, the lists and dt (synchronising)."""
         cnt = self.counts()
-        names = ["POS_L", "VEL_L", "ACC_L", "RHO_L", "P_L", "STATE_L", "AGE_L",
-                 "POS_H", "VEL_H", "ACC_H", "RHO_H", "P_H", "STATE_H",
-                 "LIST_L", "LIST_H", "LIST_G", "DT"]
+        names = ["POS_L", "VEL_L", "ACC_L", "RHO_L", "P_L", "VIS_L", "STATE_L",
+                 "AGE_L", "POS_H", "VEL_H", "ACC_H", "RHO_H", "P_H", "VIS_H",
+                 "STATE_H", "LIST_L", "LIST_H", "LIST_G", "DT"]
         snap = {n: self.read(n, cnt) for n in names}
         snap["counts"] = cnt
         return snap
