@@ -445,9 +445,11 @@ static float child_radius(float m_l, float rho)
 }
 
 /* R's adjustPosition (R:1007-1022): clamp the child's distance to its
- * parent at 0.75*sqrt(m/(rho pi)); the reference evaluates r and r/dist in
- * double (min(float,double) is double). A coincident child (dist 0, where
- * the reference divides 0/0) is left in place. */
+ * parent at 0.75*sqrt(m/(rho pi)). The bound is a double (m/(rho*PI) with
+ * the double PI, R:1018), the minimum min(float, double) is a double, and
+ * it is stored to `float r`; then r/dist and the product with posji are
+ * FP32 (R:1020-1021), each operation rounded. A coincident child (dist 0,
+ * where the reference divides 0/0) is left in place. */
 static void clamp_child(float* ph, const float* pl, float m_l, float rho)
 {
     float dx = ph[0] - pl[0];
@@ -459,10 +461,12 @@ static void clamp_child(float* ph, const float* pl, float m_l, float rho)
         return;
     }
     double rmax = 0.75 * sqrt((double)m_l / ((double)rho * ORC_PI));
-    double r = (double)dist < rmax ? (double)dist : rmax;
-    double t = r / (double)dist;
-    ph[0] = (float)((double)pl[0] + t * (double)dx);
-    ph[1] = (float)((double)pl[1] + t * (double)dy);
+    float r = (float)((double)dist < rmax ? (double)dist : rmax);
+    float t = r / dist;
+    float ox = t * dx;
+    float oy = t * dy;
+    ph[0] = pl[0] + ox;
+    ph[1] = pl[1] + oy;
 }
 
 void orc_adapt(const orc_params* p, orc_state* s, const orc_grid* g,

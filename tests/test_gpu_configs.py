@@ -10,6 +10,9 @@ high-resolution cap, and host writes that break the list invariants.
 * C2 with a cap that binds on the first step: the admitted splits are the
   first candidates in sorted order (the oracle's rule), the cap is filled
   and never exceeded.
+* Children pulled beyond their clamp radius: clamped bit-exactly as the
+  oracle does.
+* Host writes that break the list invariants are refused before a launch.
 
 Each case prints its name before touching the GPU.
 """
@@ -123,6 +126,31 @@ def test_binding_cap_parity():
             check_step(sol, sc.cfg, sc.bnd_xy, rep)
             assert sol.counts()["n_high"] <= 2048
         print(rep, flush=True)
+    finally:
+        sol.close()
+
+
+def test_binding_clamp_parity():
+    """Children pulled beyond the clamp radius of their parent: the step
+    brings them back (R/WCSPHSolver.cu:1007-1022, FP32 r / dist), bit-exact
+    with the oracle."""
+    from twoscalesplit_amd import MODE_REFERENCE
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_REFERENCE, split_x=0.08)
+    _say(sc.name, "children beyond the clamp radius")
+    sol = _solver(sc)
+    try:
+        sol.step(2)
+        groups = sol.read("LIST_G")
+        assert len(groups) >= 8
+        ph, pl, rho = sol.read("POS_H"), sol.read("POS_L"), sol.read("RHO_L")
+        m = sc.cfg.mass_low
+        for q, p in enumerate(groups[:8]):
+            rmax = 0.75 * np.sqrt(m / (float(rho[p]) * np.pi))
+            k = 4 * int(p) + (q % 4)
+            d = np.float32([np.cos(q), np.sin(q)])
+            ph[k] = (pl[p] + np.float32(1.5 * rmax) * d).astype(np.float32)
+        sol.write("POS_H", ph)
+        check_step(sol, sc.cfg, sc.bnd_xy, [])
     finally:
         sol.close()
 

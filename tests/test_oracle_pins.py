@@ -518,6 +518,24 @@ def test_split_geometry_and_clamp():
         par = o.a["pos_l"][p].astype(np.float64)
         rmax = 0.75 * math.sqrt(m / (float(o.a["rho_l"][p]) * math.pi))
         assert np.all(np.linalg.norm(ch - par, axis=1) <= rmax * (1 + 1e-6))
+    # a child pulled far from its parent is brought back to rmax along the
+    # same direction; its siblings inside rmax do not move (R:1007-1022)
+    p = int(o.a["list_g"][0])
+    par = o.a["pos_l"][p].copy()
+    rho = float(o.a["rho_l"][p])
+    rmax = 0.75 * math.sqrt(m / (rho * math.pi))
+    far = (par + np.float32(3.0 * rmax) * np.float32([0.6, 0.8])).astype(np.float32)
+    before = o.a["pos_h"][4 * p:4 * p + 4].copy()
+    o.a["pos_h"][4 * p + 2] = far
+    g = o.grid()
+    o.adapt(g, o.a["pos_l"].copy(), o.a["pos_h"].copy())
+    got = o.a["pos_h"][4 * p + 2].astype(np.float64)
+    d = got - par.astype(np.float64)
+    assert np.linalg.norm(d) == pytest.approx(rmax, rel=1e-6)
+    assert abs(d[0] * 0.8 - d[1] * 0.6) <= 1e-6 * rmax
+    for k in (0, 1, 3):
+        if np.linalg.norm(before[k].astype(np.float64) - par) < rmax * (1 - 1e-6):
+            np.testing.assert_array_equal(o.a["pos_h"][4 * p + k], before[k])
 
 
 def test_lists_stay_consistent_under_band_rules():

@@ -878,11 +878,11 @@ __global__ void k_split_cand(DevState d, const uint32_t* __restrict__ vals) {
     }
 }
 
-// R's adjustPosition (R:1007-1022): r and r/dist in double (the reference's
-// min(float, double)); a coincident child is left in place. rmax =
-// 0.75 sqrt(m_l / (rho pi)) in double, the same for a group's four
-// children (clamp_radius). Where dist < rmax, r / dist = dist / dist = 1
-// exactly, so the division is taken only for a child that is clamped.
+// R's adjustPosition (R:1007-1022): the bound rmax = 0.75 sqrt(m_l /
+// (rho pi)) is a double (double pi), the same for a group's four children
+// (clamp_radius); min(float dist, double rmax) is stored to a float r, and
+// r / dist and the offsets are FP32, each operation rounded (the build does
+// not contract). A coincident child is left in place.
 __device__ __forceinline__ double clamp_radius(float m_l, float rho) {
     return 0.75 * sqrt((double)m_l / ((double)rho * 3.14159265358979323846));
 }
@@ -894,9 +894,11 @@ __device__ __forceinline__ float2 clamp_child(float2 ph, float2 pl, double rmax)
     float b = dy * dy;
     float dist = sqrtf(a + b);
     if (!(dist > 0.0f)) return ph;
-    const double t = (double)dist < rmax ? 1.0 : rmax / (double)dist;
-    return make_float2((float)((double)pl.x + t * (double)dx),
-                       (float)((double)pl.y + t * (double)dy));
+    const float r = (double)dist < rmax ? dist : (float)rmax;
+    const float t = r / dist;
+    const float ox = t * dx;
+    const float oy = t * dy;
+    return make_float2(pl.x + ox, pl.y + oy);
 }
 
 // Existing groups: age, promotion (dormant test of R:1053-1095, enabled by
