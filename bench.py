@@ -14,6 +14,12 @@ Euler, adaptivity + list compaction, FP64 reductions (dt, conservation).
 value = particle-steps/s = steps * (n_low + n_high) / wall time, with the
 state resident in HBM.
 
+Beside the headline: sections for the plain (C1), two-scale (C2) and
+natural-band large (C3) dam breaks, each with GPU and CPU particle-steps/s,
+p50/p99 step latency and the error against the FP64 oracle; the batched
+(C4) and served (C5) scene mixes against the same solver run
+scene-at-a-time.
+
 Prints ONE JSON line. --gpus must be 1 (one GPU per box).
 """
 from __future__ import annotations
@@ -22,8 +28,14 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
+
+# the CPU legs' OpenMP threads stay on one core each, packed (read by the
+# OpenMP runtime when it starts, before any parallel region)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 import numpy as np
 
@@ -77,21 +89,37 @@ def _traffic(kernel: str) -> dict | None:
     return None
 
 
-def _cpu_model() -> str:
+def host_cpu() -> dict:
+    """The host's CPU as the CPU legs use it: model (lscpu), logical CPUs,
+    the CPUs this process may run on, and the OpenMP threads of the legs
+    (every CPU of the job's share: its affinity mask, bounded by the
+    OMP_NUM_THREADS the job is given)."""
+    model = "unknown"
     try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
         pass
-    return "unknown"
+    aff = len(os.sched_getaffinity(0))
+    threads = aff
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads": omp or None, "threads": threads,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
 
 
-def cpu_baseline(snap: dict, sc, steps: int, threads: int) -> dict:
+def cpu_baseline(snap: dict, sc, steps: int, cpu: dict) -> dict:
     """The oracle's FP32 transcription of the reference kernels (OpenMP)
-    on the GPU state of the same workload, on `threads` host threads and
-    (one step) on one."""
+    on the GPU state of the same workload, on every thread of the job's
+    CPU share and (one step) on one."""
     from oracle import binding as ob
+    threads = cpu["threads"]
 
     def rate(nsteps: int, nthreads: int) -> tuple[float, float, int]:
         o = ob.Scene(ob.params_from_config(sc.cfg), sc.cap_low, sc.bnd_xy)
@@ -112,7 +140,123 @@ def cpu_baseline(snap: dict, sc, steps: int, threads: int) -> dict:
                       f"transcription of the reference kernels, OpenMP",
             "seconds": round(dt, 3),
             "value_1_thread": one, "seconds_1_thread": round(dt1, 3),
-            "cpu_model": _cpu_model()}
+            **cpu}
+
+
+def oracle_errors(pre: dict, post: dict, sc) -> dict:
+    """CPU leg: the GPU's step from `pre` to `post` against the FP64 oracle
+    re-seeded from `pre` -- max relative density error, pressure error
+    scaled by max(|p|, B) and by max(|p|, 7 B 2^-23) (p50/p99/max), and the
+    acceleration error on the GPU's density (scaled by max(|a|, sum of the
+    term magnitudes))."""
+    from oracle import binding as ob
+    o = ob.Scene(ob.params_from_config(sc.cfg), sc.cap_low, sc.bnd_xy)
+    o.load_snapshot(pre)
+    g = o.grid()
+    rl, pl, rh, ph = o.density64(g)
+    c = pre["counts"]
+    lists = (pre["LIST_L"][:c["n_low"]], pre["LIST_H"][:c["n_high"]])
+    B = float(sc.cfg.tait_coefficient)
+    er, ep, epr = [], [], []
+    for ids, r64, p64, R, P in ((lists[0], rl, pl, "RHO_L", "P_L"),
+                                (lists[1], rh, ph, "RHO_H", "P_H")):
+        if len(ids) == 0:
+            continue
+        rg = post[R][ids].astype(np.float64)
+        pg = post[P][ids].astype(np.float64)
+        er.append(np.abs(rg - r64[ids]) / np.abs(r64[ids]))
+        d = np.abs(pg - p64[ids])
+        ep.append(d / np.maximum(np.abs(p64[ids]), B))
+        epr.append(d / np.maximum(np.abs(p64[ids]), 7.0 * B * 2.0 ** -23))
+    for k, v in (("rho_l", "RHO_L"), ("p_l", "P_L"), ("rho_h", "RHO_H"),
+                 ("p_h", "P_H")):
+        o.a[k][:] = post[v]
+    al, ah, sl, sh = o.force64(g)
+    ea = []
+    for ids, a64, s, A in ((lists[0], al, sl, "ACC_L"),
+                           (lists[1], ah, sh, "ACC_H")):
+        if len(ids) == 0:
+            continue
+        d = np.linalg.norm(post[A][ids].astype(np.float64) - a64[ids], axis=1)
+        ea.append(d / np.maximum(np.linalg.norm(a64[ids], axis=1), s[ids]))
+    cat = lambda v: np.concatenate(v) if v else np.zeros(1)
+    er, ep, epr, ea = cat(er), cat(ep), cat(epr), cat(ea)
+    pct = lambda v: [float(np.percentile(v, 50)), float(np.percentile(v, 99)),
+                     float(v.max())]
+    return {"rho_rel_max": float(er.max()), "p_rel_B_max": float(ep.max()),
+            "p_rel_floor_p50_p99_max": pct(epr),
+            "acc_rel_max": float(ea.max()),
+            "tolerance": "1e-5 on rho, p (B-scaled) and acc"}
+
+
+def run_section(sc, steps: int, latency_steps: int) -> tuple[dict, tuple]:
+    """One scene, alone, on the GPU: particle-steps/s over `steps`
+    graph-replayed steps from the initial state (the state resident in
+    HBM), the time-averaged particle counts, p50/p99 of synchronised single
+    steps; returns the section and the (pre, post) states of one more step
+    for the CPU leg's oracle check."""
+    from twoscalesplit_amd import Solver
+    sol = Solver(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+    try:
+        sol.synchronize()
+        c0 = sol.counts()
+        t0 = time.perf_counter()
+        sol.step_graph(steps)
+        sol.synchronize()
+        t1 = time.perf_counter()
+        c1 = sol.counts()
+        ps = c1["particle_steps"] - c0["particle_steps"]
+        lat = []
+        for _ in range(latency_steps):
+            a = time.perf_counter()
+            sol.step(1)
+            sol.synchronize()
+            lat.append((time.perf_counter() - a) * 1e3)
+        pre = sol.snapshot()
+        sol.step(1)
+        post = sol.snapshot()
+        cfin = sol.counts()
+    finally:
+        sol.close()
+    out = {
+        "workload": sc.name,
+        "steps": steps,
+        "value": round(ps / (t1 - t0), 1),
+        "unit": "particle-steps/s",
+        "ms_per_step": round((t1 - t0) / steps * 1e3, 4),
+        "particles_per_step_mean": round(ps / steps, 1),
+        "n_low": c1["n_low_last"],
+        "n_high_mean": round(ps / steps - c1["n_low_last"], 1),
+        "n_high_at_end": cfin["n_high_last"],
+        "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                            "p99": round(float(np.percentile(lat, 99)), 4)},
+    }
+    return out, (pre, post)
+
+
+def section_cpu_leg(out: dict, sc, states: tuple, cpu_steps: int, cpu: dict,
+                    cpu_1thread_steps: int) -> None:
+    """CPU leg of a section: the oracle check of the GPU's step, then the
+    oracle's FP32 transcription of the reference kernels timed from the
+    same initial state on the job's CPU threads (and a shorter run on
+    one). Runs after every GPU measurement: the OpenMP runtime binds the
+    calling thread when it starts."""
+    out["oracle_error"] = oracle_errors(states[0], states[1], sc)
+    from oracle import binding as ob
+
+    def cpu_run(nsteps: int, threads: int) -> dict:
+        o = ob.Scene.from_initial(sc.cfg, sc.low_xy, sc.cap_low, sc.bnd_xy)
+        ps0 = 0
+        t = time.perf_counter()
+        for _ in range(nsteps):
+            ps0 += o.s.n_l + o.s.n_h
+            o.step(32, threads)
+        dt = time.perf_counter() - t
+        return {"value": round(ps0 / dt, 1), "steps": nsteps,
+                "seconds": round(dt, 3), "threads": threads}
+    out["cpu"] = cpu_run(cpu_steps, cpu["threads"])
+    out["cpu_1_thread"] = cpu_run(cpu_1thread_steps, 1)
+    out["speedup_vs_cpu"] = round(out["value"] / out["cpu"]["value"], 1)
 
 
 def run_batched(args, rule: str) -> dict:
@@ -141,6 +285,7 @@ def run_batched(args, rule: str) -> dict:
     t2 = time.perf_counter()
     cnt = [b.counts(i) for i in ids]
     ps_total = sum(c["particle_steps"] for c in cnt)
+    n_low_total = sum(c["n_low_last"] for c in cnt)
     # post-tuning particle-steps: total minus the tuning steps' share
     lat = []
     for _ in range(args.latency_steps):
@@ -183,6 +328,7 @@ def run_batched(args, rule: str) -> dict:
         "autotune_steps": tune["steps_taken"],
         "tuning": tune["tuning"],
         "n_low": n_low,
+        "n_high_mean": round(ps_total / args.batch_steps - n_low_total, 1),
         "n_high_at_end": n_high,
         "particle_steps": ps_total,
         "step_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
@@ -215,12 +361,18 @@ def main() -> int:
     ap.add_argument("--copy-probe-mb", type=int, default=512,
                     help="plain-copy bandwidth probe size (0: skip); also "
                          "the reference dispatch for HBM counter calibration")
+    ap.add_argument("--sections", default="c1,c2,c3band",
+                    help="comma list of per-config sections beside the "
+                         "headline (c1, c2, c3band; empty: none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
                           "sharding is design-only (--gpus must be 1)"}))
         return 2
     _ensure_built()
+    # the CPU share, read before the OpenMP runtime starts (it binds the
+    # calling thread)
+    cpu = host_cpu()
     from twoscalesplit_amd import Solver, scenes
 
     _log("scene")
@@ -302,6 +454,46 @@ def main() -> int:
     gb = grid_bytes(n_now, cells, passes)
     grid_gbs = gb / (ph["grid"] * 1e-3) / 1e9
     traffic = _traffic(kernel_name)
+    # every neighbour kernel and the grid build against its attainable
+    # rate min(HBM peak x AI, FP32 peak), AI = algorithmic flops / bytes;
+    # bytes per level (SURVEY's compulsory-traffic formulas): density
+    # N 17 + 8 C + N_other 9, force + integration N 49 + 8 C + N_other 25
+    # + N_B 8; counter bytes from the committed PMC summary
+    nl_, nh_ = c1["n_low_last"], c1["n_high_last"]
+    cl_, ch_ = c1["cells_low"], c1["cells_high"]
+    nb_ = c1["n_bnd"]
+    alg_bytes = {"density_low": nl_ * 17 + 8 * cl_ + nh_ * 9,
+                 "density_high": nh_ * 17 + 8 * ch_ + nl_ * 9,
+                 "force_low": nl_ * 49 + 8 * cl_ + nh_ * 25 + nb_ * 8,
+                 "force_high": nh_ * 49 + 8 * ch_ + nl_ * 25 + nb_ * 8}
+    kern_names = {"density_low": "k_density<false>",
+                  "density_high": "k_density<true>",
+                  "force_low": "k_force<false>", "force_high": "k_force<true>"}
+    kernels = {}
+    for k in fl:
+        ms = ph[k]
+        ai = fl[k] / alg_bytes[k]
+        att = min(HBM_PEAK_GBS * 1e9 * ai, FP32_PEAK_TFLOPS * 1e12)
+        tr = _traffic(kern_names[k])
+        kernels[k] = {
+            "kernel": kern_names[k], "ms": round(ms, 4),
+            "tflops": round(fl[k] / (ms * 1e-3) / 1e12, 3),
+            "algorithmic_gbs": round(alg_bytes[k] / (ms * 1e-3) / 1e9, 1),
+            "ai_flop_per_byte": round(ai, 1),
+            "attainable_tflops": round(att / 1e12, 1),
+            "frac_of_attainable": round(fl[k] / (ms * 1e-3) / att, 4),
+            "counter_bytes": tr["bytes"] if tr else None,
+            "counter_gbs": round(tr["bytes"] / (ms * 1e-3) / 1e9, 1) if tr else None,
+        }
+    # the grid build's counter bytes: hash, one histogram, scan and scatter
+    # per radix pass, reorder
+    gtr = {kn: _traffic(kn) for kn in ("k_begin_hash", "k_sort_hist", "k_sort_scan",
+                                       "k_sort_scatter", "k_reorder")}
+    grid_counter = None
+    if all(gtr.values()):
+        grid_counter = (gtr["k_begin_hash"]["bytes"] + gtr["k_reorder"]["bytes"] +
+                        passes * (gtr["k_sort_hist"]["bytes"] + gtr["k_sort_scan"]["bytes"] +
+                                  gtr["k_sort_scatter"]["bytes"]))
     roofline = {
         "kernel": kernel_name,
         "bound": "valu",
@@ -351,7 +543,11 @@ def main() -> int:
             "frac": round(grid_gbs / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": int(gb),
             "radix_passes": passes,
+            "counter_bytes": grid_counter,
+            "counter_gbs": (round(grid_counter / (ph["grid"] * 1e-3) / 1e9, 1)
+                            if grid_counter else None),
         },
+        "kernels": kernels,
         "work": {k: int(v) for k, v in work.items()},
     }
     if args.copy_probe_mb > 0:
@@ -363,6 +559,26 @@ def main() -> int:
         out["copy_probe_gbs"] = round(g.value, 1)
     snap = None if args.no_cpu_baseline else sol.snapshot()
     sol.close()
+    secs = [s for s in args.sections.split(",") if s]
+    cpu_legs = []
+    if secs:
+        out["configs"] = {}
+    for name in secs:
+        # C1: 1000 fixed-dt steps on the GPU and on the CPU transcription;
+        # C2: 2000 steps; C3 band: 500 GPU steps and a bounded CPU sample
+        if name == "c1":
+            sc1, st, cs, c1t = scenes.c1_plain(), 1000, 1000, 100
+        elif name == "c2":
+            sc1, st, cs, c1t = scenes.c2_two_scale(), 2000, 2000, 100
+        elif name == "c3band":
+            sc1, st, cs, c1t = scenes.c3_large(forced=False), 500, 10, 1
+        else:
+            raise SystemExit(f"unknown section {name!r}")
+        _log("section", name, st, "steps")
+        sec, states = run_section(sc1, st, args.latency_steps)
+        out["configs"][name] = sec
+        if not args.no_cpu_baseline:
+            cpu_legs.append((sec, sc1, states, cs, c1t))
     if args.batch_scenes > 0:
         out["batched"] = run_batched(args, "band")
         out["batched_reference_rule"] = run_batched(args, "reference")
@@ -372,10 +588,12 @@ def main() -> int:
              args.serve_rate, "/s")
         out["served"] = serve.run_rate(scenes.SEED, args.serve_rate,
                                        args.serve_requests, 8_000_000)
+    for sec, sc1, states, cs, c1t in cpu_legs:
+        _log("cpu leg", sec["workload"])
+        section_cpu_leg(sec, sc1, states, cs, cpu, c1t)
     if snap is not None:
-        threads = min(16, os.cpu_count() or 1)
-        _log("cpu baseline", args.cpu_steps, "steps,", threads, "threads")
-        out["cpu_baseline"] = cpu_baseline(snap, sc, args.cpu_steps, threads)
+        _log("cpu baseline", args.cpu_steps, "steps,", cpu["threads"], "threads")
+        out["cpu_baseline"] = cpu_baseline(snap, sc, args.cpu_steps, cpu)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out, default=float), flush=True)
