@@ -374,6 +374,17 @@ int tss_frame(tss_solver* s, uint32_t fields, void* dst, size_t cap,
               size_t* bytes);
 int tss_batch_frame(tss_batch* b, int scene_id, uint32_t fields, void* dst,
                     size_t cap, size_t* bytes);
+/* tss_batch_frame without any allocation: the frame is staged in a fixed
+ * device array of the library (64 MiB, one per device, held until the
+ * process ends) instead of a buffer grown with hipMalloc. No counterpart in
+ * the reference; it is for callers that cannot allocate device memory --
+ * tight serving loops and harnesses that forbid hipMalloc. A frame whose
+ * staging (its sections plus 4 bytes per 1024 list entries of the batch)
+ * exceeds 64 MiB is refused with TSS_E_INVALID. Every call on a device
+ * shares that device's array: calls that overlap on one GPU (from several
+ * threads or handles) race and must be serialised by the caller. */
+int tss_batch_frame_noalloc(tss_batch* b, int scene_id, uint32_t fields,
+                            void* dst, size_t cap, size_t* bytes);
 
 /* ---- scene snapshot / resume ---------------------------------------------- */
 /* A self-contained image of one scene between steps: its configuration,

@@ -219,6 +219,7 @@ PROTOTYPES = {
     "tss_batch_restore": (_I, [_P, _P, _SZ, C.POINTER(_I)]),
     "tss_frame": (_I, [_P, C.c_uint32, _P, _SZ, C.POINTER(_SZ)]),
     "tss_batch_frame": (_I, [_P, _I, C.c_uint32, _P, _SZ, C.POINTER(_SZ)]),
+    "tss_batch_frame_noalloc": (_I, [_P, _I, C.c_uint32, _P, _SZ, C.POINTER(_SZ)]),
     "tss_batch_conservation": (_I, [_P, _I, _P, _P]),
     "tss_batch_synchronize": (_I, [_P]),
     "tss_batch_num_scenes": (_I, [_P, C.POINTER(_I)]),

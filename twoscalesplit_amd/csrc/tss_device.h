@@ -272,6 +272,17 @@ int frame_tiles(int n);
 // (at most n_out entries written); enqueued on stream.
 void launch_frame_pack(const DevState& d, const int* list, int n, int lo, int hi,
                        int* tile_cnt, int n_out, const FrameOut& out, void* stream);
+// A scene's boundary positions by id: out[bids[k]] = bpos[k], k < nb (its
+// cell-ordered copy and scene-local ids).
+void launch_frame_bnd(const float2* bpos, const int* bids, int nb, float2* out,
+                      void* stream);
+// Fixed device scratch of the frame export that allocates nothing: a
+// file-scope array of kNoAllocScratchBytes in the code object, one copy per
+// device, held until the process ends. noalloc_scratch gives its address on
+// `device` (looked up once per device with that device current; host code
+// uses this address, never the array's name).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+hipError_t noalloc_scratch(int device, void** out);
 void launch_dam_break(float2* pos, int nx, int ny, float dx, float jitter,
                       unsigned long long seed, unsigned long long scene,
                       void* stream);

@@ -80,6 +80,10 @@ int guarded(F&& f) {
 void require(bool ok, const std::string& msg, int code = TSS_E_INVALID) {
     if (!ok) throw TssError(code, msg);
 }
+// A literal message builds no string unless the check fails.
+void require(bool ok, const char* msg, int code = TSS_E_INVALID) {
+    if (!ok) throw TssError(code, msg);
+}
 
 // Largest non-negative float f with sqrtf(f) <= R (inclusive) or < R: the
 // squared-distance threshold equivalent to the FP32 predicate on
@@ -225,6 +229,7 @@ struct tss_batch {
         {256, 16, 0, 0}, {256, 16, 0, 0}, {256, 16, 0, 0}, {256, 16, 0, 0}};
     long long cap_l = 0, cap_b = 0, ccl = 0, cch = 0;
     int max_scenes = 0;
+    int device = 0;                 // the device the batch lives on
     uint32_t* nb_counts = nullptr;  // neighbour inspection scratch
     uint32_t* nb_sums = nullptr;
     size_t nb_cap = 0;
@@ -286,6 +291,7 @@ struct tss_batch {
         ccl = ds.cells_low;
         cch = ds.cells_high;
         max_scenes = ds.max_scenes;
+        HIP_CHECK(hipGetDevice(&device));
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         const long long CL = cap_l, U = 5 * CL;
         d.CL = (int)CL;
@@ -1258,6 +1264,22 @@ struct tss_batch {
 
     // ---- frame export ------------------------------------------------------
     void frame(int sid, uint32_t fields, void* dst, size_t cap, size_t* bytes) {
+        frame_impl(sid, fields, dst, cap, bytes, nullptr, 0);
+    }
+
+    // The same frame staged in the fixed device scratch: no allocation.
+    void frame_noalloc(int sid, uint32_t fields, void* dst, size_t cap, size_t* bytes) {
+        void* p = nullptr;
+        HIP_CHECK(tss::noalloc_scratch(device, &p));
+        frame_impl(sid, fields, dst, cap, bytes, (char*)p, tss::kNoAllocScratchBytes);
+    }
+
+    // A frame of scene sid into the host buffer dst. Staging: the batch's
+    // own buffer, grown on demand (fixed == nullptr), or the `fixed` device
+    // buffer of fixed_cap bytes, where the frame must fit. Staging layout:
+    // low sections | high sections | boundary section | tile counts.
+    void frame_impl(int sid, uint32_t fields, void* dst, size_t cap, size_t* bytes,
+                    char* fixed, size_t fixed_cap) {
         require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
         require((fields & ~(uint32_t)TSS_FRAME_ALL) == 0, "unknown frame field bits");
         const SceneHost& s = scenes[sid];
@@ -1266,21 +1288,34 @@ struct tss_batch {
         const size_t size = tss_frame_size(fields, sc.n_l, sc.n_h, nb);
         *bytes = size;
         if (!dst) return;
-        require(cap >= size, "frame buffer too small (" + std::to_string(size) + " bytes needed)");
+        if (cap < size)
+            throw TssError(TSS_E_INVALID, "frame buffer too small (" + std::to_string(size) +
+                                              " bytes needed)");
         const tss::DevCounts c = read_counts();
         const size_t lb = frame_level_bytes(fields, sc.n_l);
         const size_t hb = frame_level_bytes(fields, sc.n_h);
+        const size_t bb = frame_pad(8 * (size_t)nb);
         const int tl_n = tss::frame_tiles(c.n_l) + 1, th_n = tss::frame_tiles(c.n_h) + 1;
-        const size_t need = frame_pad(lb + hb) + 4 * (size_t)(tl_n + th_n);
-        if (frame_cap < need) {
-            if (frame_buf) (void)hipFree(frame_buf);
-            frame_buf = nullptr;
-            frame_cap = 0;
-            HIP_CHECK(hipMalloc(&frame_buf, need));
-            frame_cap = need;
+        const size_t need = lb + hb + bb + 4 * (size_t)(tl_n + th_n);
+        char* stage;
+        if (fixed) {
+            if (need > fixed_cap)
+                throw TssError(TSS_E_INVALID,
+                               "frame needs " + std::to_string(need) +
+                                   " bytes of device staging; the no-allocation export holds at most " +
+                                   std::to_string(fixed_cap) + " (use tss_batch_frame)");
+            stage = fixed;
+        } else {
+            if (frame_cap < need) {
+                if (frame_buf) (void)hipFree(frame_buf);
+                frame_buf = nullptr;
+                frame_cap = 0;
+                HIP_CHECK(hipMalloc(&frame_buf, need));
+                frame_cap = need;
+            }
+            stage = (char*)frame_buf;
         }
-        char* stage = (char*)frame_buf;
-        int* tl = (int*)(stage + frame_pad(lb + hb));
+        int* tl = (int*)(stage + lb + hb + bb);
         int* th = tl + tl_n;
         auto sections = [&](char* base, size_t n) {
             tss::FrameOut o{};
@@ -1302,17 +1337,20 @@ struct tss_batch {
         };
         const int par = sh.list_parity;
         const int CL = d.CL;
-        // padding bytes of the sections are zero
-        if (lb + hb) HIP_CHECK(hipMemsetAsync(stage, 0, lb + hb, stream));
+        // padding bytes of the sections are zero (a fill kernel: sections
+        // are multiples of 16 bytes)
+        if (lb + hb + bb) tss::launch_fill_int((int*)stage, 0, (long long)((lb + hb + bb) / 4), stream);
         tss::launch_frame_pack(d, d.list_l[par], c.n_l, s.lbase, s.lbase + s.lcap, tl, sc.n_l,
                                sections(stage, sc.n_l), stream);
         tss::launch_frame_pack(d, d.list_h[par], c.n_h, CL + 4 * s.lbase,
                                CL + 4 * (s.lbase + s.lcap), th, sc.n_h,
                                sections(stage + lb, sc.n_h), stream);
+        if (nb) tss::launch_frame_bnd(d.bpos + s.bbase, d.bids + s.bbase, nb,
+                                      (float2*)(stage + lb + hb), stream);
         char* out = (char*)dst;
         int tot[2] = {0, 0};
-        if (lb + hb)
-            HIP_CHECK(hipMemcpyAsync(out + sizeof(tss_frame_header), stage, lb + hb,
+        if (lb + hb + bb)
+            HIP_CHECK(hipMemcpyAsync(out + sizeof(tss_frame_header), stage, lb + hb + bb,
                                      hipMemcpyDeviceToHost, stream));
         if (c.n_l) HIP_CHECK(hipMemcpyAsync(&tot[0], tl + tl_n - 1, 4, hipMemcpyDeviceToHost, stream));
         if (c.n_h) HIP_CHECK(hipMemcpyAsync(&tot[1], th + th_n - 1, 4, hipMemcpyDeviceToHost, stream));
@@ -1321,7 +1359,6 @@ struct tss_batch {
         HIP_CHECK(hipStreamSynchronize(stream));
         require(tot[0] == sc.n_l && tot[1] == sc.n_h, "frame selection disagrees with the scene counts",
                 TSS_E_STATE);
-        if (nb) read_field(sid, TSS_POS_B, out + sizeof(tss_frame_header) + lb + hb, 8 * (size_t)nb);
         tss_frame_header hd{};
         hd.magic = TSS_FRAME_MAGIC;
         hd.version = TSS_FRAME_VERSION;
@@ -1336,8 +1373,6 @@ struct tss_batch {
             hd.domain_end[k] = s.cfg.domain_end[k];
         }
         std::memcpy(out, &hd, sizeof(hd));
-        const size_t bpad = frame_pad(8 * (size_t)nb) - 8 * (size_t)nb;
-        if (bpad) std::memset(out + size - bpad, 0, bpad);
     }
 
     // ---- snapshot / resume -------------------------------------------------
@@ -2111,6 +2146,14 @@ int tss_frame(tss_solver* s, uint32_t fields, void* dst, size_t cap, size_t* byt
     return guarded([&] {
         require(s && bytes, "null argument");
         s->b.frame(s->sid, fields, dst, cap, bytes);
+    });
+}
+
+int tss_batch_frame_noalloc(tss_batch* b, int sid, uint32_t fields, void* dst, size_t cap,
+                            size_t* bytes) {
+    return guarded([&] {
+        require(b && bytes, "null argument");
+        b->frame_noalloc(sid, fields, dst, cap, bytes);
     });
 }
 

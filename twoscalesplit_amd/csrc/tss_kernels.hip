@@ -15,6 +15,7 @@
 
 #include <climits>
 #include <cstdint>
+#include <mutex>
 #include <type_traits>
 
 #include "tss_device.h"
@@ -1742,6 +1743,46 @@ __global__ __launch_bounds__(kFrameThreads) void k_frame_pack(
         base += tot;
         __syncthreads();
     }
+}
+
+__global__ void k_frame_bnd(const float2* __restrict__ bpos, const int* __restrict__ bids,
+                            int nb, float2* __restrict__ out) {
+    GRID_STRIDE(k, nb) {
+        const int i = bids[k];
+        if (i >= 0 && i < nb) out[i] = bpos[k];
+    }
+}
+
+void launch_frame_bnd(const float2* bpos, const int* bids, int nb, float2* out,
+                      void* stream) {
+    if (nb <= 0) return;
+    k_frame_bnd<<<blocks_for(nb, 256, 1024), 256, 0, (hipStream_t)stream>>>(bpos, bids, nb, out);
+}
+
+// Staging of the frame export that allocates nothing. Its address differs
+// per device and is known to host code only through hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int device, void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* ptr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        ptr[device] = p;
+    }
+    *out = ptr[device];
+    return hipSuccess;
 }
 
 void launch_dam_break(float2* pos, int nx, int ny, float dx, float jitter,
