@@ -114,16 +114,26 @@ __device__ __forceinline__ Box scan_box(const SceneDev& sc, bool high, float2 x,
 
 // Contiguous sorted range covering cells x0..x1 of one grid row (cells are
 // row-major, so neighbouring cells of a row are adjacent in sorted order).
+// The entries of four cells are loaded together, then combined in order.
 __device__ __forceinline__ void row_run(const int* __restrict__ cs,
                                         const int* __restrict__ ce, int row,
                                         int x0, int x1, int& s, int& e) {
     s = -1;
     e = -1;
-    for (int cx = x0; cx <= x1; ++cx) {
-        int st = cs[row + cx];
-        if (st >= 0) {
-            if (s < 0) s = st;
-            e = ce[row + cx];
+    for (int c0 = x0; c0 <= x1; c0 += 4) {
+        int st[4], en[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int cx = row + min(c0 + k, x1);
+            st[k] = cs[cx];
+            en[k] = ce[cx];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (c0 + k <= x1 && st[k] >= 0) {
+                if (s < 0) s = st[k];
+                e = en[k];
+            }
         }
     }
 }
