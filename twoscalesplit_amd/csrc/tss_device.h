@@ -28,6 +28,11 @@ constexpr int kMaxPasses = (32 + kRadixBits - 1) / kRadixBits;  // 32-bit keys
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort tile
+// Batches whose resident scenes hold at most kSmallSortKeys keys (5 keys per
+// low-id capacity) sort in one workgroup of kSmallSortThreads threads, every
+// radix pass in one launch.
+constexpr int kSmallSortThreads = 1024;
+constexpr long long kSmallSortKeys = 65536;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // entries per compaction tile
@@ -225,6 +230,8 @@ struct StepShape {
     int clear_blocks;         // blocks of the cell-table clear (clear_blk)
     int cap_binds;            // 1: a resident scene has cap_high < 4 cap_low
     int band_any;             // 1: a resident scene uses the surface band rule
+    int small_sort;           // 1: the resident capacity (5 cap_alive keys)
+                              // fits kSmallSortKeys: one-workgroup sort
 };
 
 // Timing points of one step (events recorded before / after each phase):

@@ -464,6 +464,7 @@ struct tss_batch {
         sh.scan_tiles_l = (int)((cap + tss::kScanTile - 1) / tss::kScanTile);
         sh.scan_tiles_h = (int)((4 * cap + tss::kScanTile - 1) / tss::kScanTile);
         sh.scan_tiles_g = sh.scan_tiles_l;
+        sh.small_sort = U <= tss::kSmallSortKeys ? 1 : 0;
         sh.cap_binds = 0;
         sh.band_any = 0;
         for (const SceneHost& s : scenes) {

@@ -514,6 +514,108 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     }
 }
 
+// The same stable LSD sort for small batches, in one workgroup and one
+// launch: per pass a histogram of the live keys (LDS atomics), the digit
+// bases (one wave), then the keys in order, in chunks of one key per thread,
+// each ranked like an item of k_sort_scatter (kRadixBits ballots within the
+// wave, the earlier waves' counts of its digit through LDS). Passes read
+// keys/vals[pass & 1] and write the other buffer, as the multi-tile sort
+// does, so the result is in buffer (passes & 1) and identical to it.
+__global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevState d, int passes) {
+    constexpr int kWaves = kSmallSortThreads / 64;
+    __shared__ uint32_t hcount[kRadix];
+    __shared__ uint32_t base[kRadix];
+    __shared__ uint32_t running[kRadix];
+    __shared__ uint32_t wcnt[kWaves][kRadix];
+    const int n = d.counts->n_l + d.counts->n_h;
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int w = t >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int i = t; i < kWaves * kRadix; i += kSmallSortThreads) (&wcnt[0][0])[i] = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = kRadixBits * pass;
+        const uint32_t* __restrict__ kin = d.keys[pass & 1];
+        const uint32_t* __restrict__ vin = d.vals[pass & 1];
+        uint32_t* __restrict__ kout = d.keys[(pass & 1) ^ 1];
+        uint32_t* __restrict__ vout = d.vals[(pass & 1) ^ 1];
+        for (int i = t; i < kRadix; i += kSmallSortThreads) {
+            hcount[i] = 0;
+            running[i] = 0;
+        }
+        __syncthreads();
+        for (int e = t; e < n; e += kSmallSortThreads)
+            atomicAdd(&hcount[(kin[e] >> shift) & (kRadix - 1u)], 1u);
+        __syncthreads();
+        // digit bases: exclusive scan of the histogram by the first wave,
+        // kRadix / 64 digits per lane
+        if (w == 0) {
+            constexpr int kPer = kRadix / 64;
+            uint32_t loc[kPer], sum = 0;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                loc[q] = hcount[lane * kPer + q];
+                sum += loc[q];
+            }
+            uint32_t incl = sum;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t a = (uint32_t)__shfl_up((int)incl, off, 64);
+                incl += lane >= off ? a : 0u;
+            }
+            uint32_t run = incl - sum;
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) {
+                base[lane * kPer + q] = run;
+                run += loc[q];
+            }
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < n; c0 += kSmallSortThreads) {
+            const int e = c0 + t;
+            const bool valid = e < n;
+            const uint32_t key = valid ? kin[e] : 0u;
+            const uint32_t val = valid ? vin[e] : 0u;
+            const uint32_t dg = (key >> shift) & (kRadix - 1u);
+            uint64_t m = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < kRadixBits; ++b) {
+                const uint64_t bb = __ballot((dg >> b) & 1u);
+                m &= ((dg >> b) & 1u) ? bb : ~bb;
+            }
+            const uint32_t rk = (uint32_t)__popcll(m & lt);
+            const bool leader = valid && rk == 0;
+            if (leader) wcnt[w][dg] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t off = 0, tot = 0;
+            bool last = true;
+            if (leader) {
+                for (int v = 0; v < kWaves; ++v) {
+                    const uint32_t c = wcnt[v][dg];
+                    off += v < w ? c : 0u;
+                    tot += c;
+                    last = last && (v <= w || c == 0u);
+                }
+                off += running[dg];
+            }
+            __syncthreads();
+            if (leader) {
+                if (last) running[dg] += tot;  // the digit's last wave this chunk
+                wcnt[w][dg] = 0;
+            }
+            const int src = valid ? __ffsll((long long)m) - 1 : lane;
+            off = (uint32_t)__shfl((int)off, src, 64);
+            if (valid) {
+                const uint32_t dst = base[dg] + off + rk;
+                kout[dst] = key;
+                vout[dst] = val;
+            }
+        }
+        // the pass's writes are read by the next pass (same workgroup)
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Cell ranges (R:1300-1340) + reorder into cell-sorted SoA + scene segments.
 // ---------------------------------------------------------------------------
@@ -1822,7 +1924,11 @@ static int enqueue_grid(const DevState& d, const StepShape& sh,
     k_begin_hash<<<sh.clear_blocks + sh.grid_elem, 256, 0, st>>>(
         d, sh.clear_blocks, d.list_l[par], d.list_h[par], d.keys[0], d.vals[0]);
     int cur = 0;
-    for (int pass = 0; pass < sh.passes; ++pass) {
+    if (sh.small_sort) {
+        k_sort_small<<<1, kSmallSortThreads, 0, st>>>(d, sh.passes);
+        cur = sh.passes & 1;
+    }
+    for (int pass = 0; !sh.small_sort && pass < sh.passes; ++pass) {
         const int shift = kRadixBits * pass;
         k_sort_hist<<<sh.sort_grid, kSortThreads, 0, st>>>(
             d.keys[cur], d.counts, shift, d.hist, sh.sort_tiles);
