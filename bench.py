@@ -276,7 +276,7 @@ def run_batched(args, rule: str) -> dict:
     b.synchronize()
     # the autotuner's trials are the run's first steps
     t0 = time.perf_counter()
-    tune = b.autotune(steps_per_trial=1, rounds=2)
+    tune = b.autotune(steps_per_trial=1, rounds=3)
     b.synchronize()
     t1 = time.perf_counter()
     rest = args.batch_steps - tune["steps_taken"]
@@ -382,7 +382,7 @@ def main() -> int:
     # shapes on this workload (its trials are ordinary steps), then both
     # graph variants (plain and timed) are built
     _log("warmup + autotune")
-    tune = sol.autotune(steps_per_trial=1, rounds=2)
+    tune = sol.autotune(steps_per_trial=1, rounds=3)
     sol.step_graph(args.warmup)
     sol.enable_timing(True)
     sol.step_graph(max(4, args.warmup // 4))

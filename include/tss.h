@@ -299,8 +299,9 @@ int tss_batch_set_tuning(tss_batch* b, const tss_tuning* t);
  * After one ordinary step, stage 1 tries every block x depth, stage 2 the
  * XCD order and persistent grid caps (4 and 8 blocks per CU) on each
  * kernel's stage-1 winner; each candidate runs `rounds` times for
- * steps_per_trial steps, in alternating order, and per kernel the lowest
- * mean time per target of its level is kept (and returned in out).
+ * steps_per_trial steps, in alternating order; a candidate's time per
+ * target of the kernel's level is its fastest trial, and per kernel the
+ * fastest candidate is kept (and returned in out).
  * steps_taken: steps advanced. trial_log (optional): log_rows x 8 floats per
  * trial, (block, depth, xcd_order, grid, ms of kernels 0..3); n_log rows. */
 int tss_batch_autotune(tss_batch* b, int steps_per_trial, int rounds,

@@ -87,6 +87,8 @@ struct SceneDev {
     int relax_iters;          // relaxation iterations of new children
     float relax_frac;         // relaxation step = relax_frac * dt
     int hh_inner;             // 1: 2H <= h, so r <= H gives q_h = 2r/h <= 1
+    int hh_half;              // 1: H = h/2 exactly: sig_H2 = 4 sig_h2,
+                              // gc_H2 = 8 gc_h2, q_H = 2 q_h in FP32
 };
 
 struct SceneCount {

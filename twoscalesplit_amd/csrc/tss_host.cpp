@@ -599,6 +599,10 @@ struct tss_batch {
         s.relax_iters = c.relax_iters;
         s.relax_frac = c.relax_dt_frac;
         s.hh_inner = 2.0 * (double)s.H <= (double)s.h ? 1 : 0;
+        s.hh_half = s.hh_inner && 2.0f * s.H == s.h && s.sig_H2 == 4.0f * s.sig_h2 &&
+                            s.gc_H2 == 8.0f * s.gc_h2 && s.q_H == 2.0f * s.q_h
+                        ? 1
+                        : 0;
         return s;
     }
 
@@ -990,11 +994,12 @@ struct tss_batch {
         // per-target time of each kernel: the population changes from step to
         // step (the first step of a scene splits before any child exists),
         // so a trial's kernel time is divided by the targets of its level in
-        // that step; trials whose level had no target do not count
+        // that step; trials whose level had no target do not count. Each
+        // candidate keeps its fastest trial (the other trials carry the
+        // interference of whatever else the device did meanwhile)
         auto run_trials = [&](const std::vector<std::array<tss::PairTune, tss::kPairKernels>>& cand,
                               std::vector<std::array<double, tss::kPairKernels>>& tot) {
-            tot.assign(cand.size(), {0.0, 0.0, 0.0, 0.0});
-            std::vector<std::array<int, tss::kPairKernels>> cnt(cand.size(), {0, 0, 0, 0});
+            tot.assign(cand.size(), {1e300, 1e300, 1e300, 1e300});
             for (int r = 0; r < rounds; ++r) {
                 for (size_t i = 0; i < cand.size(); ++i) {
                     // alternate the visiting order so drift cancels
@@ -1010,8 +1015,7 @@ struct tss_batch {
                                                         dc.n_h_last};
                     for (int k = 0; k < tss::kPairKernels; ++k) {
                         if (ntg[k] <= 0) continue;
-                        tot[c][k] += (double)ph[1 + k] / (double)ntg[k];
-                        ++cnt[c][k];
+                        tot[c][k] = std::min(tot[c][k], (double)ph[1 + k] / (double)ntg[k]);
                     }
                     if (trial_log && logged < log_cap) {
                         float* row = trial_log + (size_t)logged * 8;
@@ -1025,9 +1029,6 @@ struct tss_batch {
                     }
                 }
             }
-            for (size_t c = 0; c < cand.size(); ++c)
-                for (int k = 0; k < tss::kPairKernels; ++k)
-                    tot[c][k] = cnt[c][k] ? tot[c][k] / cnt[c][k] : 1e300;
         };
         // one ordinary step first: trials then start from a populated state
         set_timing(false);

@@ -173,13 +173,41 @@ __device__ __forceinline__ void m4_blend_inner(const SceneDev& sc, float r, floa
     g = fmaf(sc.gc_h2, dh, sc.gc_H2 * dH);
 }
 
-// m4_blend, or m4_blend_inner when INNER (chosen per scene, hh_inner).
-template <bool INNER>
+// W_bar and G of a high target's own-level pairs (r <= H) when H = h/2
+// exactly, so that sigma_H = 4 sigma_h, g_H = 8 g_h and q_H = 2 q_h hold in
+// FP32 (hh_half): the two M4 kernels of R:116-239 summed and expanded in
+// q = 2r/h <= 1 and u = 1 - q,
+//   W_bar = sigma_h/2 [(1 + 3u + 3u^2 + 29u^3) - 16 max(2u - 1, 0)^3],
+//   G = g_h/2 q (68 - 99 q) for q < 1/2, g_h/2 (1 + u (2 + 29 u)) above,
+// each form where it has no cancellation (ten and seven operations against
+// sixteen and thirteen for the two kernels separately; as accurate in FP32).
+__device__ __forceinline__ float wbar_half(const SceneDev& sc, float r) {
+    const float u = fmaf(-r, sc.q_h, 1.0f);
+    const float p = fmaf(fmaf(fmaf(29.0f, u, 3.0f), u, 3.0f), u, 1.0f);
+    const float v = fmaxf(fmaf(2.0f, u, -1.0f), 0.0f);
+    return sc.sig_h2 * fmaf(-16.0f, v * v * v, p);
+}
+
+__device__ __forceinline__ float gbar_half(const SceneDev& sc, float r) {
+    const float q = r * sc.q_h;
+    const float u = 1.0f - q;
+    const float ga = q * fmaf(-99.0f, q, 68.0f);
+    const float gb = fmaf(u, fmaf(29.0f, u, 2.0f), 1.0f);
+    return sc.gc_h2 * (q < 0.5f ? ga : gb);
+}
+
+// Kernel forms, chosen per scene: KM 0 m4_blend, 1 m4_blend_inner (hh_inner),
+// 2 wbar_half / gbar_half (hh_half).
+template <int KM>
 __device__ __forceinline__ void m4_pick(const SceneDev& sc, float r, float& w, float& g) {
-    if (INNER)
+    if (KM == 2) {
+        w = wbar_half(sc, r);
+        g = gbar_half(sc, r);
+    } else if (KM == 1) {
         m4_blend_inner(sc, r, w, g);
-    else
+    } else {
         m4_blend(sc, r, w, g);
+    }
 }
 
 // Wall weight Gamma(r)/(mass factor) (R:66-114), qs = 2/ell.

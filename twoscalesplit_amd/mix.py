@@ -25,7 +25,7 @@ def run(n_scenes: int, steps: int, rule: str, tune: bool = True) -> dict:
         ids = [b.add_scene(s.cfg, s.low_xy, s.cap_low, s.bnd_xy) for s in scs]
         b.synchronize()
         t0 = time.perf_counter()
-        taken = b.autotune(steps_per_trial=1, rounds=2)["steps_taken"] if tune else 0
+        taken = b.autotune(steps_per_trial=1, rounds=3)["steps_taken"] if tune else 0
         b.step_graph(max(steps - taken, 0))
         b.synchronize()
         t1 = time.perf_counter()
