@@ -512,6 +512,13 @@ uint64_t tss_splitmix64_next(uint64_t* state);
  * roofline and of the HBM counter calibration. No counterpart in the
  * reference. */
 int tss_bandwidth_probe(size_t bytes, int reps, double* gbs);
+/* Sparse-read rate: one 8-byte read every stride_bytes over a fresh
+ * `bytes`-sized buffer (each read its own request), reps passes; gbs = 8-byte
+ * words read / time. With a buffer larger than the caches and a stride of a
+ * cache line or more, every read fetches its own line: the reference of the
+ * HBM counter calibration for gathered, smaller-than-line requests. No
+ * counterpart in the reference. */
+int tss_gather_probe(size_t bytes, int stride_bytes, int reps, double* gbs);
 
 const char* tss_last_error(void);
 const char* tss_version(void);

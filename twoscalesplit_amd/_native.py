@@ -236,6 +236,7 @@ PROTOTYPES = {
                                  _I, _I, _P, _P]),
     "tss_splitmix64_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
     "tss_bandwidth_probe": (_I, [_SZ, _I, C.POINTER(C.c_double)]),
+    "tss_gather_probe": (_I, [_SZ, _I, _I, C.POINTER(C.c_double)]),
     "tss_last_error": (C.c_char_p, []),
     "tss_version": (C.c_char_p, []),
 }

@@ -296,6 +296,9 @@ void launch_dam_break(float2* pos, int nx, int ny, float dx, float jitter,
                       unsigned long long seed, unsigned long long scene,
                       void* stream);
 void launch_copy16(const void* src, void* dst, long long n16, void* stream);
+// n 8-byte reads, one every `stride` 8-byte words of src (src holds at least
+// n * stride words); sink: one 8-byte word.
+void launch_gather_probe(const void* src, long long n, int stride, void* sink, void* stream);
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
                          void* stream);
 

@@ -2554,6 +2554,48 @@ int tss_bandwidth_probe(size_t bytes, int reps, double* gbs) {
     });
 }
 
+int tss_gather_probe(size_t bytes, int stride_bytes, int reps, double* gbs) {
+    return guarded([&] {
+        require(gbs && reps >= 1 && stride_bytes >= 8 && stride_bytes % 8 == 0 &&
+                    bytes >= (size_t)stride_bytes,
+                "invalid argument");
+        const int stride = stride_bytes / 8;
+        const long long n = (long long)(bytes / (size_t)stride_bytes);
+        void *a = nullptr, *sink = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (a) (void)hipFree(a);
+            if (sink) (void)hipFree(sink);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+        };
+        try {
+            HIP_CHECK(hipMalloc(&a, (size_t)n * (size_t)stride_bytes));
+            HIP_CHECK(hipMalloc(&sink, 8));
+            HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            // every byte 0x5A: the words XOR to 0 or 0x5A...5A, never the
+            // probe's sink trigger
+            HIP_CHECK(hipMemsetAsync(a, 0x5A, (size_t)n * (size_t)stride_bytes, st));
+            tss::launch_gather_probe(a, n, stride, sink, st);  // warm
+            HIP_CHECK(hipEventRecord(e0, st));
+            for (int i = 0; i < reps; ++i) tss::launch_gather_probe(a, n, stride, sink, st);
+            HIP_CHECK(hipEventRecord(e1, st));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            *gbs = 8.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
 const char* tss_last_error(void) { return g_last_error.c_str(); }
 
 const char* tss_version(void) { return "tss 0.1.0 gfx950"; }

@@ -1761,6 +1761,24 @@ void launch_copy16(const void* src, void* dst, long long n16, void* stream) {
         (const float4*)src, (float4*)dst, n16);
 }
 
+// Sparse reads: one 8-byte load every `stride` 8-byte words of src (n
+// loads), each lane its own address -- the reference workload of the HBM
+// counter calibration for requests smaller than a cache line. The sink is
+// written only if a value equals `never`, which the caller's fill excludes.
+__global__ void k_gather_probe(const uint64_t* __restrict__ src, long long n, int stride,
+                               uint64_t never, uint64_t* __restrict__ sink) {
+    uint64_t acc = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        acc ^= src[i * stride];
+    if (acc == never) sink[0] = acc;
+}
+
+void launch_gather_probe(const void* src, long long n, int stride, void* sink, void* stream) {
+    k_gather_probe<<<blocks_for(n, 256, 8192), 256, 0, (hipStream_t)stream>>>(
+        (const uint64_t*)src, n, stride, 0x5A5A5A5A5A5A5A5Bull, (uint64_t*)sink);
+}
+
 // Seeded dam-break block on the device: particle idx = j * nx + i at
 // ((0.5 + i) dx, (0.5 + j) dx) + (2u - 1) * amp per axis, u = the top 24 bits
 // of SplitMix64 outputs keyed by (seed, scene, idx) -- the same arithmetic,
