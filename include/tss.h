@@ -251,6 +251,21 @@ typedef struct tss_tuning {
     int xcd_order[4];
     int grid[4];
 } tss_tuning;
+/* Sort of the grid build (R/WCSPHSolver.cu:2828-2834, thrust::sort_by_key
+ * by cell, stable): TSS_SORT_AUTO picks per batch shape; TSS_SORT_RADIX is
+ * the 8-bit LSD radix sort (hist / scan / scatter per pass);
+ * TSS_SORT_COUNT the counting sort by cell (counts, prefix over cells,
+ * scatter, rank within each cell by list position); TSS_SORT_SMALL the
+ * one-workgroup radix sort, for resident capacities of at most 13107 low
+ * ids (65536 keys; TSS_E_INVALID otherwise). AUTO takes SMALL where it fits,
+ * else COUNT when the resident cells number at most 4 per key of capacity,
+ * else RADIX. Every path gives the same order, cell ranges and results. */
+#define TSS_SORT_AUTO 0
+#define TSS_SORT_RADIX 1
+#define TSS_SORT_COUNT 2
+#define TSS_SORT_SMALL 3
+int tss_set_grid_sort(tss_solver* s, int path);
+int tss_batch_set_grid_sort(tss_batch* b, int path);
 /* Launch shapes and autotuner of a single-scene solver (as for batches,
  * tss_batch_autotune below). */
 int tss_get_tuning(tss_solver* s, tss_tuning* out);

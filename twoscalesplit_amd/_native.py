@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib",
 
 TSS_OK = 0
 MODE_PLAIN, MODE_REFERENCE, MODE_BAND = 0, 1, 2
+# grid-build sort paths (tss_set_grid_sort)
+SORT_AUTO, SORT_RADIX, SORT_COUNT, SORT_SMALL = 0, 1, 2, 3
 DT_FIXED, DT_CFL = 0, 1
 
 FIELDS = [
@@ -193,6 +195,8 @@ PROTOTYPES = {
     "tss_enable_timing": (_I, [_P, _I]),
     "tss_phase_times": (_I, [_P, _P]),
     "tss_work_counts": (_I, [_P, _P]),
+    "tss_set_grid_sort": (_I, [_P, _I]),
+    "tss_batch_set_grid_sort": (_I, [_P, _I]),
     "tss_get_tuning": (_I, [_P, C.POINTER(TssTuning)]),
     "tss_set_tuning": (_I, [_P, C.POINTER(TssTuning)]),
     "tss_autotune": (_I, [_P, _I, _I, C.POINTER(TssTuning), C.POINTER(_I), _P,

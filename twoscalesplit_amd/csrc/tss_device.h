@@ -33,6 +33,13 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // keys per sort tile
 // radix pass in one launch.
 constexpr int kSmallSortThreads = 1024;
 constexpr long long kSmallSortKeys = 65536;
+// Counting sort by cell (the grid build when the resident cells number at
+// most kCountCellsPerKey per key): cell counts are scanned in tiles of
+// kCellTile cells (kCellThreads threads, kCellTile / kCellThreads cells
+// each).
+constexpr int kCellThreads = 256;
+constexpr int kCellTile = 1024;
+constexpr int kCountCellsPerKey = 4;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;  // entries per compaction tile
@@ -153,7 +160,12 @@ struct DevState {
     uint32_t* digit_tot;      // [kRadix] digit totals of the current pass
     int* cstart;
     int* cend;                // [CCL + CCH]
-    int* cscene;              // [CCL + CCH] scene owning each cell
+    int* cscene;              // [CCL + CCH] scene owning each cell (-1: none)
+    int* ccount;              // [CCL + CCH] keys per cell (counting sort;
+                              // zero between grid builds)
+    int* ctile;               // [CCL + CCH / kCellTile + 1] cell tile sums
+    uint2* cslot;             // [5 CL] counting sort: (list position, id)
+                              // per slot of a cell's range
     // [cells / kClearChunk + 2 S + 1] cell ranges [x, y) of the resident
     // scenes, at most kClearChunk cells each: one block of the step's
     // cell-table clear per entry
@@ -234,6 +246,12 @@ struct StepShape {
     int band_any;             // 1: a resident scene uses the surface band rule
     int small_sort;           // 1: the resident capacity (5 cap_alive keys)
                               // fits kSmallSortKeys: one-workgroup sort
+    int count_sort;           // 1: counting sort by cell (not small_sort,
+                              // key_cells <= kCountCellsPerKey * 5 cap_alive)
+    int key_cells;            // cells [0, key_cells) hold every resident cell
+    int cell_tiles;           // ceil(key_cells / kCellTile)
+    int sorted_buf;           // keys/vals buffer holding the sorted order
+    int solo;                 // the only resident scene's slot, else -1
 };
 
 // Timing points of one step (events recorded before / after each phase):

@@ -229,6 +229,7 @@ struct tss_batch {
         {256, 16, 0, 0}, {256, 16, 0, 0}, {256, 16, 0, 0}, {256, 16, 0, 0}};
     long long cap_l = 0, cap_b = 0, ccl = 0, cch = 0;
     int max_scenes = 0;
+    int sort_path = TSS_SORT_AUTO;  // grid-build sort (tss_set_grid_sort)
     int device = 0;                 // the device the batch lives on
     uint32_t* nb_counts = nullptr;  // neighbour inspection scratch
     uint32_t* nb_sums = nullptr;
@@ -273,6 +274,9 @@ struct tss_batch {
         b += 3 * (ds.cells_low + ds.cells_high) + ds.cells_low;  // cell state and surface flags
         b += 2 * tss::kFlagPad;
         b += 4 * (ds.cells_low + ds.cells_high);  // cell -> scene
+        // counting sort: counts per cell, tile sums, slots
+        b += 4 * (ds.cells_low + ds.cells_high) + 4 * ((ds.cells_low + ds.cells_high) / tss::kCellTile + 1);
+        b += 8 * U;
         b += 8 * ((ds.cells_low + ds.cells_high) / tss::kClearChunk + 2LL * ds.max_scenes + 1);
         b += 16 * 4 * CL;                         // relaxation scratch (when used)
         return b;
@@ -323,6 +327,9 @@ struct tss_batch {
         d.digit_tot = dalloc<uint32_t>(tss::kRadix, owned);
         d.cstart = dalloc<int>(ccl + cch, owned);
         d.cscene = dalloc<int>(ccl + cch, owned);
+        d.ccount = dalloc<int>(ccl + cch, owned);
+        d.ctile = dalloc<int>((ccl + cch) / tss::kCellTile + 1, owned);
+        d.cslot = dalloc<uint2>(U, owned);
         d.cend = dalloc<int>(ccl + cch, owned);
         d.bstart = dalloc<int>(ccl, owned);
         d.bend = dalloc<int>(ccl, owned);
@@ -378,6 +385,8 @@ struct tss_batch {
         HIP_CHECK(hipMemsetAsync(d.age, 0, sizeof(int) * CL, stream));
         HIP_CHECK(hipMemsetAsync(d.merged, 0, (size_t)CL, stream));
         HIP_CHECK(hipMemsetAsync(d.cand, 0, (size_t)CL, stream));
+        tss::launch_fill_int(d.cscene, -1, ccl + cch, stream);
+        HIP_CHECK(hipMemsetAsync(d.ccount, 0, sizeof(int) * (size_t)(ccl + cch), stream));
         tss::launch_fill_int(d.cstart, -1, ccl + cch, stream);
         tss::launch_fill_int(d.cend, -1, ccl + cch, stream);
         tss::launch_fill_int(d.bstart, -1, ccl, stream);
@@ -464,7 +473,10 @@ struct tss_batch {
         sh.scan_tiles_l = (int)((cap + tss::kScanTile - 1) / tss::kScanTile);
         sh.scan_tiles_h = (int)((4 * cap + tss::kScanTile - 1) / tss::kScanTile);
         sh.scan_tiles_g = sh.scan_tiles_l;
-        sh.small_sort = U <= tss::kSmallSortKeys ? 1 : 0;
+        // a forced one-workgroup sort that no longer fits (scenes added
+        // since) falls back to the automatic choice
+        const bool small_fits = U <= tss::kSmallSortKeys;
+        sh.small_sort = small_fits && (sort_path == TSS_SORT_SMALL || sort_path == TSS_SORT_AUTO);
         sh.cap_binds = 0;
         sh.band_any = 0;
         for (const SceneHost& s : scenes) {
@@ -478,6 +490,20 @@ struct tss_batch {
         while ((1LL << bits) < maxkey) ++bits;
         sh.key_bits = bits;
         sh.passes = (bits + tss::kRadixBits - 1) / tss::kRadixBits;
+        sh.key_cells = (int)maxkey;
+        sh.cell_tiles = (int)((maxkey + tss::kCellTile - 1) / tss::kCellTile);
+        sh.count_sort = sort_path == TSS_SORT_COUNT ||
+                        (sort_path == TSS_SORT_AUTO && !sh.small_sort &&
+                         maxkey <= tss::kCountCellsPerKey * U);
+        sh.sorted_buf = sh.count_sort ? 1 : (sh.passes & 1);
+        sh.solo = -1;
+        int resident = 0;
+        for (int k = 0; k < max_scenes; ++k)
+            if (scenes[k].alive) {
+                ++resident;
+                sh.solo = k;
+            }
+        if (resident != 1) sh.solo = -1;
         for (int k = 0; k < tss::kPairKernels; ++k) sh.tune[k] = tune[k];
         sh.relax_iters = 0;
         for (auto& s : scenes)
@@ -800,6 +826,8 @@ struct tss_batch {
         HIP_CHECK(hipMemcpyAsync(d.scenes + sid, &sd, sizeof(sd), hipMemcpyHostToDevice, stream));
         tss::launch_fill_int(d.bstart + s.cbl, -1, s.ncl, stream);
         tss::launch_fill_int(d.bend + s.cbl, -1, s.ncl, stream);
+        tss::launch_fill_int(d.cscene + s.cbl, -1, s.ncl, stream);
+        tss::launch_fill_int(d.cscene + s.cbh, -1, s.nch, stream);
         HIP_CHECK(hipStreamSynchronize(stream));
         a_low.release(s.lbase, s.lcap);
         a_bnd.release(s.bbase, s.nb);
@@ -1109,10 +1137,10 @@ struct tss_batch {
             case TSS_LIST_L: return {d.list_l[sh.list_parity], 4, 0, 1};
             case TSS_LIST_H: return {d.list_h[sh.list_parity], 4, 0, 2};
             case TSS_LIST_G: return {d.list_g[sh.list_parity], 4, 0, 11};
-            case TSS_SORTED_L: return {d.vals[sh.passes & 1], 4, 0, 3};
-            case TSS_SORTED_H: return {d.vals[sh.passes & 1], 4, 0, 4};
-            case TSS_KEYS_L: return {d.keys[sh.passes & 1], 4, 0, 5};
-            case TSS_KEYS_H: return {d.keys[sh.passes & 1], 4, 0, 6};
+            case TSS_SORTED_L: return {d.vals[sh.sorted_buf], 4, 0, 3};
+            case TSS_SORTED_H: return {d.vals[sh.sorted_buf], 4, 0, 4};
+            case TSS_KEYS_L: return {d.keys[sh.sorted_buf], 4, 0, 5};
+            case TSS_KEYS_H: return {d.keys[sh.sorted_buf], 4, 0, 6};
             case TSS_CELL_START_L: return {d.cstart + s.cbl, 4, (size_t)s.ncl, 7};
             case TSS_CELL_END_L: return {d.cend + s.cbl, 4, (size_t)s.ncl, 7};
             case TSS_CELL_START_H: return {d.cstart + s.cbh, 4, (size_t)s.nch, 8};
@@ -2281,6 +2309,24 @@ int tss_batch_autotune(tss_batch* b, int steps_per_trial, int rounds, tss_tuning
         if (out) tune_to_abi(b->tune, out);
         if (steps_taken) *steps_taken = taken;
     });
+}
+
+int tss_batch_set_grid_sort(tss_batch* b, int path) {
+    return guarded([&] {
+        require(b, "null argument");
+        require(path >= TSS_SORT_AUTO && path <= TSS_SORT_SMALL, "unknown sort path");
+        long long cap = 0;
+        for (const auto& sc : b->scenes)
+            if (sc.alive) cap += sc.lcap;
+        require(path != TSS_SORT_SMALL || 5 * std::max<long long>(cap, 1) <= tss::kSmallSortKeys,
+                "the one-workgroup sort holds at most 65536 keys (13107 low ids)");
+        b->sort_path = path;
+        b->update_shape();
+    });
+}
+
+int tss_set_grid_sort(tss_solver* s, int path) {
+    return s ? tss_batch_set_grid_sort(&s->b, path) : tss_batch_set_grid_sort(nullptr, path);
 }
 
 int tss_get_tuning(tss_solver* s, tss_tuning* out) {

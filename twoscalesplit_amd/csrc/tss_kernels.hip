@@ -289,8 +289,13 @@ __global__ void k_fill_int(int* __restrict__ p, int v, long long n) {
 // the resident scenes (block b the cells of range clear_blk[b]) and reset
 // the per-scene step accumulators; the other blocks hash (R:1246-1298):
 // key = unified cell index of the particle. The two parts touch disjoint
-// data; the cell tables are next written by k_reorder, after the sort.
-__global__ void k_begin_hash(DevState d, int nclear,
+// data; the cell tables are next written by the grid build's last kernels.
+// count = 1 (counting sort by cell): the cell ranges are rewritten whole by
+// k_cell_ranges, so they are not cleared here; each key is counted in its
+// cell (one atomic per run of equal keys along a wave's lanes) and the ids
+// are not copied (the scatter reads them from the lists). solo: the slot of
+// the only resident scene, or -1 (the scene then comes from scene_of).
+__global__ void k_begin_hash(DevState d, int nclear, int count, int solo,
                              const int* __restrict__ list_l,
                              const int* __restrict__ list_h,
                              uint32_t* __restrict__ keys,
@@ -298,8 +303,10 @@ __global__ void k_begin_hash(DevState d, int nclear,
     if ((int)blockIdx.x < nclear) {
         const int2 r = d.clear_blk[blockIdx.x];
         for (int i = r.x + (int)threadIdx.x; i < r.y; i += (int)blockDim.x) {
-            d.cstart[i] = -1;
-            d.cend[i] = -1;
+            if (!count) {
+                d.cstart[i] = -1;
+                d.cend[i] = -1;
+            }
             d.cflag[2 * i] = 0;
             d.cflag[2 * i + 1] = 0;
             d.csurf[i] = 0;
@@ -326,24 +333,310 @@ __global__ void k_begin_hash(DevState d, int nclear,
     const int nl = d.counts->n_l;
     const int n = nl + d.counts->n_h;
     const int stride = (gridDim.x - nclear) * blockDim.x;
-    for (int k = (blockIdx.x - nclear) * blockDim.x + threadIdx.x; k < n; k += stride) {
-        int u = k < nl ? list_l[k] : list_h[k - nl];
-        bool high = u >= d.CL;
-        int s = d.scene_of[low_of(d, u)];
-        const SceneDev& sc = d.scenes[s];
-        float2 x = d.pos[u];
-        uint32_t key;
-        if (!high) {
-            int cx = axis_cell(x.x, 0.0f, sc.ox, sc.h, sc.dlx);
-            int cy = axis_cell(x.y, 0.0f, sc.oy, sc.h, sc.dly);
-            key = (uint32_t)(sc.cbase_l + cy * sc.dlx + cx);
-        } else {
-            int cx = axis_cell(x.x, 0.0f, sc.ox, sc.H, sc.dhx);
-            int cy = axis_cell(x.y, 0.0f, sc.oy, sc.H, sc.dhy);
-            key = (uint32_t)(sc.cbase_h + cy * sc.dhx + cx);
+    const int lane = threadIdx.x & 63;
+    // block-uniform trip count: the run votes below see whole waves, whose
+    // valid lanes are a prefix
+    for (int b0 = (blockIdx.x - nclear) * blockDim.x; b0 < n; b0 += stride) {
+        const int k = b0 + (int)threadIdx.x;
+        const bool valid = k < n;
+        uint32_t key = 0u;
+        if (valid) {
+            const int u = k < nl ? list_l[k] : list_h[k - nl];
+            const bool high = u >= d.CL;
+            // with one resident scene, no lookup by id
+            const int s = solo >= 0 ? solo : d.scene_of[low_of(d, u)];
+            const SceneDev& sc = d.scenes[s];
+            const float2 x = d.pos[u];
+            if (!high) {
+                int cx = axis_cell(x.x, 0.0f, sc.ox, sc.h, sc.dlx);
+                int cy = axis_cell(x.y, 0.0f, sc.oy, sc.h, sc.dly);
+                key = (uint32_t)(sc.cbase_l + cy * sc.dlx + cx);
+            } else {
+                int cx = axis_cell(x.x, 0.0f, sc.ox, sc.H, sc.dhx);
+                int cy = axis_cell(x.y, 0.0f, sc.oy, sc.H, sc.dhy);
+                key = (uint32_t)(sc.cbase_h + cy * sc.dhx + cx);
+            }
+            keys[k] = key;
+            if (!count) vals[k] = (uint32_t)u;
         }
-        keys[k] = key;
-        vals[k] = (uint32_t)u;
+        if (count) {
+            const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
+            const bool start = valid && (lane == 0 || prev != key);
+            const uint64_t starts = __ballot(start);
+            const int nvalid = __popcll(__ballot(valid));
+            if (start) {
+                const uint64_t later = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+                const int next = later ? __ffsll((long long)later) - 1 : nvalid;
+                atomicAdd(&d.ccount[key], next - lane);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Counting sort by cell (replaces thrust::sort_by_key, R:2828-2834, with the
+// same stable order: by cell, ties by list position):
+//   k_begin_hash (count = 1) counts the keys per cell ->
+//   k_cell_sums (per tile of kCellTile cells) -> k_cell_top (one block) ->
+//   k_cell_ranges: cstart = exclusive prefix of the counts (-1 for an empty
+//   cell), cend = cstart (the scatter's fill pointer), scene segments ->
+//   k_cell_scatter: list positions into their cell's range (atomic fill,
+//   any order within a cell; cend ends at cstart + count) ->
+//   k_cell_reorder: each entry's rank among its cell's list positions gives
+//   its sorted position; then the cell-sorted copies.
+// The order within a cell is fixed by the rank, so the result is
+// deterministic and equal to the radix sort's. The rank costs one read per
+// entry of the cell, as a target's neighbour scan does per candidate.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kCellThreads) void k_cell_sums(DevState d, int ncell) {
+    __shared__ int wsum[kCellThreads / 64];
+    const int c0 = blockIdx.x * kCellTile + (int)threadIdx.x;
+    int v = 0;
+#pragma unroll
+    for (int i = 0; i < kCellTile / kCellThreads; ++i) {
+        const int c = c0 + i * kCellThreads;
+        v += c < ncell ? d.ccount[c] : 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < kCellThreads / 64; ++w) t += wsum[w];
+        d.ctile[blockIdx.x] = t;
+    }
+}
+
+// One block: exclusive scan of the tile sums in place.
+__global__ __launch_bounds__(1024) void k_cell_top(DevState d, int ntile) {
+    __shared__ int part[1024];
+    const int per = (ntile + 1023) / 1024;
+    const int t0 = (int)threadIdx.x * per;
+    int a = 0;
+    for (int i = 0; i < per; ++i)
+        if (t0 + i < ntile) a += d.ctile[t0 + i];
+    part[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int w = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += w;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - a;
+    for (int i = 0; i < per; ++i) {
+        if (t0 + i < ntile) {
+            const int v = d.ctile[t0 + i];
+            d.ctile[t0 + i] = run;
+            run += v;
+        }
+    }
+}
+
+// Per tile: the counts to LDS (coalesced), each thread's kCellTile /
+// kCellThreads consecutive cells scanned in registers, the thread totals
+// scanned over the block, the prefixes written back through LDS. Resets the
+// counts for the next grid build. A scene's low and high cells are each one
+// range of the unified cells (low ranges below CCL, high ranges above), so
+// a cell whose owner differs from its neighbour's, or that sits at either
+// side of CCL, bounds a range: it writes that scene's sorted segment bound.
+__global__ __launch_bounds__(kCellThreads) void k_cell_ranges(DevState d, int ncell) {
+    constexpr int kPer = kCellTile / kCellThreads;  // 4 cells per thread
+    // one pad word per kPer: thread t's run starts at bank (t * 5) mod 32
+    __shared__ int cells[kCellTile + kCellTile / kPer];
+    __shared__ int owner[kCellTile];  // cscene of the tile's cells
+    __shared__ int wsum[kCellThreads / 64];
+    const int t = (int)threadIdx.x;
+    const int base = blockIdx.x * kCellTile;
+    auto slot = [](int x) { return x + x / kPer; };
+    int cnt[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int x = i * kCellThreads + t;
+        const int c = base + x;
+        cnt[i] = c < ncell ? d.ccount[c] : 0;
+        cells[slot(x)] = cnt[i];
+        owner[x] = c < ncell ? d.cscene[c] : -1;
+    }
+    __syncthreads();
+    int run[kPer];
+    int tot = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        run[j] = tot;
+        tot += cells[slot(t * kPer + j)];
+    }
+    // exclusive scan of the thread totals: within the wave, then the waves
+    const int lane = t & 63;
+    int inc = tot;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) wsum[t >> 6] = inc;
+    __syncthreads();
+    int woff = d.ctile[blockIdx.x];
+    for (int w = 0; w < (t >> 6); ++w) woff += wsum[w];
+    const int toff = woff + inc - tot;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) cells[slot(t * kPer + j)] = toff + run[j];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int x = i * kCellThreads + t;
+        const int c = base + x;
+        if (c >= ncell) continue;
+        const int ex = cells[slot(x)];
+        const int v = cnt[i] > 0 ? ex : -1;
+        d.cstart[c] = v;
+        d.cend[c] = v;
+        if (cnt[i] != 0) d.ccount[c] = 0;
+        const int s = owner[x];
+        if (s >= 0) {
+            const bool high = c >= d.CCL;
+            const int sp = x > 0 ? owner[x - 1] : (c > 0 ? d.cscene[c - 1] : -1);
+            const int sn = x + 1 < kCellTile ? owner[x + 1]
+                                             : (c + 1 < ncell ? d.cscene[c + 1] : -1);
+            const bool first = c == 0 || c == d.CCL || sp != s;
+            const bool last = c == ncell - 1 || c == d.CCL - 1 || sn != s;
+            SceneCount& sct = d.scnt[s];
+            if (first) {
+                if (high) sct.seg_h0 = ex; else sct.seg_l0 = ex;
+            }
+            if (last) {
+                if (high) sct.seg_h1 = ex + cnt[i]; else sct.seg_l1 = ex + cnt[i];
+            }
+        }
+    }
+}
+
+// List positions into their cells' ranges: one atomic per run of equal keys
+// along a wave's lanes (lists are kept in cell order, so a cell's entries
+// mostly come in one run). A slot holds the entry's list position and id
+// (read here in list order); its key goes to the same slot of the sorted
+// keys: a cell's slots all hold its key, whatever the order within the cell.
+__global__ void k_cell_scatter(DevState d, const uint32_t* __restrict__ keys,
+                               const int* __restrict__ list_l,
+                               const int* __restrict__ list_h,
+                               uint32_t* __restrict__ skeys) {
+    const int nl = d.counts->n_l;
+    const int n = nl + d.counts->n_h;
+    const int stride = gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+        const int k = b0 + (int)threadIdx.x;
+        const bool valid = k < n;
+        uint32_t key = 0u;
+        int u = 0;
+        if (valid) {
+            key = keys[k];
+            u = k < nl ? list_l[k] : list_h[k - nl];
+        }
+        const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
+        const bool start = valid && (lane == 0 || prev != key);
+        const uint64_t starts = __ballot(start);
+        const int nvalid = __popcll(__ballot(valid));
+        int pos = 0;
+        if (start) {
+            const uint64_t later = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+            const int next = later ? __ffsll((long long)later) - 1 : nvalid;
+            pos = atomicAdd(&d.cend[key], next - lane);
+        }
+        // the run's leader: the highest start lane at or below this lane
+        const uint64_t upto = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int leader = upto ? 63 - __clzll((long long)upto) : lane;
+        pos = __shfl(pos, leader, 64) + (lane - leader);
+        if (valid) {
+            d.cslot[pos] = make_uint2((uint32_t)k, (uint32_t)u);
+            skeys[pos] = key;
+        }
+    }
+}
+
+// Sorted position of the entry in slot p: its cell's start plus the number
+// of the cell's list positions below its own. A cell whose slots all lie in
+// the wave's 64 is ranked through lane shuffles (the cell's lanes are one
+// run of equal sorted keys); a cell reaching past the wave's slots reads its
+// range from the cell table. Then the cell-sorted copies (as k_reorder):
+// ids, positions, velocities, states, the id -> sorted index map, scenes,
+// cell state flags, scene flags. The id's data is loaded before the rank is
+// formed, so its latency overlaps the ranking.
+__global__ void k_cell_reorder(DevState d, const uint32_t* __restrict__ skeys,
+                               uint32_t* __restrict__ svals) {
+    const int nl = d.counts->n_l;
+    const int n = nl + d.counts->n_h;
+    const int stride = gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    for (int b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+        const int wb = b0 + (int)(threadIdx.x & ~63u);  // the wave's first slot
+        const int p = wb + lane;
+        const bool valid = p < n;
+        const uint2 eu = valid ? d.cslot[p] : make_uint2(0u, 0u);
+        const int e = (int)eu.x;
+        const int u = (int)eu.y;
+        const uint32_t key = valid ? skeys[p] : 0xFFFFFFFFu;
+        uint8_t sst = 0;
+        float2 x = make_float2(0.f, 0.f), v = x;
+        if (valid) {
+            sst = d.state[u];
+            x = d.pos[u];
+            v = d.vel[u];
+        }
+        const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
+        const bool start = valid && (lane == 0 || prev != key);
+        const uint64_t starts = __ballot(start);
+        const int nvalid = __popcll(__ballot(valid));
+        const uint64_t upto = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+        const int lead = upto ? 63 - __clzll((long long)upto) : lane;
+        const uint64_t later = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+        const int next = later ? __ffsll((long long)later) - 1 : nvalid;
+        // the run reaches past the wave's slots when its first lane is 0 and
+        // the slot before the wave holds the same key, or its end is the
+        // wave's end and the slot after holds the same key
+        bool outside = false;
+        if (valid && lead == 0 && wb > 0) outside = skeys[wb - 1] == key;
+        if (valid && next == 64 && wb + 64 < n) outside = outside || skeys[wb + 64] == key;
+        // rank among the run's lanes: a pass per lane offset, up to the
+        // longest run in the wave
+        const int len = valid ? next - lead : 0;
+        int maxlen = 0;
+#pragma unroll
+        for (int b = 6; b >= 0; --b) {
+            const int c = maxlen | (1 << b);
+            if (__ballot(len >= c) != 0ull) maxlen = c;
+        }
+        int rank = 0;
+        for (int j = 0; j < maxlen; ++j) {
+            const int ej = __shfl(e, min(lead + j, 63), 64);
+            rank += (j < len && ej < e) ? 1 : 0;
+        }
+        int k = wb + lead + rank;
+        if (valid && outside) {
+            // eight of the cell's list positions in flight at a time
+            const int st = d.cstart[key], en = d.cend[key];
+            rank = 0;
+            for (int q0 = st; q0 < en; q0 += 8) {
+                uint32_t t[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) t[i] = d.cslot[min(q0 + i, en - 1)].x;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) rank += (q0 + i < en && (int)t[i] < e) ? 1 : 0;
+            }
+            k = st + rank;
+        }
+        if (!valid) continue;
+        svals[k] = (uint32_t)u;
+        d.cflag[2 * (size_t)key + (sst & 1u)] = 1;
+        d.spos[k] = x;
+        d.sxv[k] = make_float4(x.x, x.y, v.x, v.y);
+        d.sstate[k] = sst;
+        d.sidx[u] = k;
+        const int s = d.cscene[key];
+        d.sscene[k] = s;
+        if (!(sst & 1u)) {
+            if (k >= nl) d.scnt[s].prom = 1;
+            else d.scnt[s].act = 1;
+        }
     }
 }
 
@@ -1968,7 +2261,17 @@ static int enqueue_grid(const DevState& d, const StepShape& sh,
                         hipStream_t st) {
     const int par = sh.list_parity;
     k_begin_hash<<<sh.clear_blocks + sh.grid_elem, 256, 0, st>>>(
-        d, sh.clear_blocks, d.list_l[par], d.list_h[par], d.keys[0], d.vals[0]);
+        d, sh.clear_blocks, sh.count_sort, sh.solo, d.list_l[par], d.list_h[par], d.keys[0],
+        d.vals[0]);
+    if (sh.count_sort) {
+        k_cell_sums<<<sh.cell_tiles, kCellThreads, 0, st>>>(d, sh.key_cells);
+        k_cell_top<<<1, 1024, 0, st>>>(d, sh.cell_tiles);
+        k_cell_ranges<<<sh.cell_tiles, kCellThreads, 0, st>>>(d, sh.key_cells);
+        k_cell_scatter<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[0], d.list_l[par],
+                                                     d.list_h[par], d.keys[1]);
+        k_cell_reorder<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[1], d.vals[1]);
+        return 1;
+    }
     int cur = 0;
     if (sh.small_sort) {
         k_sort_small<<<1, kSmallSortThreads, 0, st>>>(d, sh.passes);

@@ -241,6 +241,11 @@ class Solver:
     def set_tuning(self, t: N.TssTuning) -> None:
         N.check(N.lib().tss_set_tuning(self._h, C.byref(t)), "set_tuning")
 
+    def set_grid_sort(self, path: int) -> None:
+        """Sort of the grid build: N.SORT_AUTO, SORT_RADIX, SORT_COUNT or
+        SORT_SMALL (tss_set_grid_sort); results do not depend on it."""
+        N.check(N.lib().tss_set_grid_sort(self._h, path), "set_grid_sort")
+
     def autotune(self, steps_per_trial: int = 1, rounds: int = 2) -> dict:
         """Native autotuner (advances the scene by real steps)."""
         t = N.TssTuning()
@@ -407,6 +412,10 @@ class Batch:
         N.check(N.lib().tss_batch_get_tuning(self._h, C.byref(t)),
                 "tss_batch_get_tuning")
         return t
+
+    def set_grid_sort(self, path: int) -> None:
+        """Sort of the grid build (tss_batch_set_grid_sort)."""
+        N.check(N.lib().tss_batch_set_grid_sort(self._h, path), "set_grid_sort")
 
     def set_tuning(self, t: N.TssTuning) -> None:
         N.check(N.lib().tss_batch_set_tuning(self._h, C.byref(t)),
