@@ -272,6 +272,22 @@ __device__ __forceinline__ float dist_from2(float r2) {
     return __builtin_amdgcn_sqrtf(r2);
 }
 
+// Streaming stores (not kept in L2 for reuse): a kernel's results, written
+// once and read by later kernels.
+__device__ __forceinline__ void nt_store(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void nt_store(float2* p, float x, float y) {
+    float* q = reinterpret_cast<float*>(p);
+    __builtin_nontemporal_store(x, q);
+    __builtin_nontemporal_store(y, q + 1);
+}
+__device__ __forceinline__ void nt_store(float4* p, float x, float y, float z, float w) {
+    float* q = reinterpret_cast<float*>(p);
+    __builtin_nontemporal_store(x, q);
+    __builtin_nontemporal_store(y, q + 1);
+    __builtin_nontemporal_store(z, q + 2);
+    __builtin_nontemporal_store(w, q + 3);
+}
+
 __device__ __forceinline__ int low_of(const DevState& d, int u) {
     return u < d.CL ? u : (u - d.CL) >> 2;
 }
@@ -377,9 +393,11 @@ __global__ void k_begin_hash(DevState d, int nclear, int count, int solo,
 // Counting sort by cell (replaces thrust::sort_by_key, R:2828-2834, with the
 // same stable order: by cell, ties by list position):
 //   k_begin_hash (count = 1) counts the keys per cell ->
-//   k_cell_sums (per tile of kCellTile cells) -> k_cell_top (one block) ->
-//   k_cell_ranges: cstart = exclusive prefix of the counts (-1 for an empty
-//   cell), cend = cstart (the scatter's fill pointer), scene segments ->
+//   k_cell_sums (per tile of kCellTile cells) ->
+//   k_cell_ranges: each block sums the earlier tiles' sums (L2-resident,
+//   one read per tile) for its base; cstart = exclusive prefix of the
+//   counts (-1 for an empty cell), cend = cstart (the scatter's fill
+//   pointer), scene segments ->
 //   k_cell_scatter: list positions into their cell's range (atomic fill,
 //   any order within a cell; cend ends at cstart + count) ->
 //   k_cell_reorder: each entry's rank among its cell's list positions gives
@@ -407,32 +425,6 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_sums(DevState d, int ncel
     }
 }
 
-// One block: exclusive scan of the tile sums in place.
-__global__ __launch_bounds__(1024) void k_cell_top(DevState d, int ntile) {
-    __shared__ int part[1024];
-    const int per = (ntile + 1023) / 1024;
-    const int t0 = (int)threadIdx.x * per;
-    int a = 0;
-    for (int i = 0; i < per; ++i)
-        if (t0 + i < ntile) a += d.ctile[t0 + i];
-    part[threadIdx.x] = a;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int w = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += w;
-        __syncthreads();
-    }
-    int run = part[threadIdx.x] - a;
-    for (int i = 0; i < per; ++i) {
-        if (t0 + i < ntile) {
-            const int v = d.ctile[t0 + i];
-            d.ctile[t0 + i] = run;
-            run += v;
-        }
-    }
-}
-
 // Per tile: the counts to LDS (coalesced), each thread's kCellTile /
 // kCellThreads consecutive cells scanned in registers, the thread totals
 // scanned over the block, the prefixes written back through LDS. Resets the
@@ -446,9 +438,18 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_ranges(DevState d, int nc
     __shared__ int cells[kCellTile + kCellTile / kPer];
     __shared__ int owner[kCellTile];  // cscene of the tile's cells
     __shared__ int wsum[kCellThreads / 64];
+    __shared__ int bsum[kCellThreads / 64];
+    __shared__ int tbase;
     const int t = (int)threadIdx.x;
     const int base = blockIdx.x * kCellTile;
     auto slot = [](int x) { return x + x / kPer; };
+    // the tile's base: the sum of the earlier tiles' sums
+    {
+        int v = 0;
+        for (int q = t; q < (int)blockIdx.x; q += kCellThreads) v += d.ctile[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((t & 63) == 0) bsum[t >> 6] = v;
+    }
     int cnt[kPer];
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
@@ -459,6 +460,11 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_ranges(DevState d, int nc
         owner[x] = c < ncell ? d.cscene[c] : -1;
     }
     __syncthreads();
+    if (t == 0) {
+        int b = 0;
+        for (int w = 0; w < kCellThreads / 64; ++w) b += bsum[w];
+        tbase = b;
+    }
     int run[kPer];
     int tot = 0;
 #pragma unroll
@@ -475,7 +481,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_ranges(DevState d, int nc
     }
     if (lane == 63) wsum[t >> 6] = inc;
     __syncthreads();
-    int woff = d.ctile[blockIdx.x];
+    int woff = tbase;
     for (int w = 0; w < (t >> 6); ++w) woff += wsum[w];
     const int toff = woff + inc - tot;
 #pragma unroll
@@ -627,8 +633,8 @@ __global__ void k_cell_reorder(DevState d, const uint32_t* __restrict__ skeys,
         if (!valid) continue;
         svals[k] = (uint32_t)u;
         d.cflag[2 * (size_t)key + (sst & 1u)] = 1;
-        d.spos[k] = x;
-        d.sxv[k] = make_float4(x.x, x.y, v.x, v.y);
+        nt_store(&d.spos[k], x.x, x.y);
+        nt_store(&d.sxv[k], x.x, x.y, v.x, v.y);
         d.sstate[k] = sst;
         d.sidx[u] = k;
         const int s = d.cscene[key];
@@ -2265,7 +2271,6 @@ static int enqueue_grid(const DevState& d, const StepShape& sh,
         d.vals[0]);
     if (sh.count_sort) {
         k_cell_sums<<<sh.cell_tiles, kCellThreads, 0, st>>>(d, sh.key_cells);
-        k_cell_top<<<1, 1024, 0, st>>>(d, sh.cell_tiles);
         k_cell_ranges<<<sh.cell_tiles, kCellThreads, 0, st>>>(d, sh.key_cells);
         k_cell_scatter<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[0], d.list_l[par],
                                                      d.list_h[par], d.keys[1]);
