@@ -66,11 +66,32 @@ def _ensure_built():
 
 
 def grid_bytes(n: int, cells: int, passes: int) -> float:
-    """Algorithmic HBM bytes of one grid build: hash (read pos, write key and
-    index), P radix passes over key+value, range read, reorder gather +
-    scatter of the 21-byte payload, cell table clear + writes."""
+    """Algorithmic HBM bytes of one grid build by the radix-sort formula:
+    hash (read pos, write key and index), P radix passes over key+value,
+    range read, reorder gather + scatter of the 21-byte payload, cell table
+    clear + writes."""
     S = 21
     return n * (16 + 16 * passes + 4 + 2 * S) + 16 * cells
+
+
+def count_sort_bytes(n: int, cells: int) -> float:
+    """Algorithmic HBM bytes of the counting-sort grid build: hash (list 4,
+    position 8, key 4), scatter (key 4, list 4, slot 8, sorted key 4),
+    reorder (slot 8, sorted key 4, gathered state + position + velocity 17;
+    writes id 4, position 8, position + velocity 16, state 1, id -> index 4,
+    scene 4); per cell: the count read twice (tile sums, ranges), the owner
+    4, the range 8, the count reset 4."""
+    return n * (16 + 20 + 66) + cells * 24
+
+
+def grid_sort_path(cap_low: int, cells: int) -> str:
+    """The grid build's sort as TSS_SORT_AUTO picks it (include/tss.h):
+    one workgroup up to 65536 keys, the counting sort when the cells number
+    at most 4 per key, else the radix sort."""
+    keys = 5 * max(cap_low, 1)
+    if keys <= 65536:
+        return "small"
+    return "count" if cells <= 4 * keys else "radix"
 
 
 def _traffic(kernel: str) -> dict | None:
@@ -449,10 +470,12 @@ def main() -> int:
                    "force_high": "k_force<true>"}[dom]
     n_now = c1["n_low"] + c1["n_high"]
     cells = c1["cells_low"] + c1["cells_high"]
-    # 8-bit radix digits over the unified cell keys
+    # 8-bit radix digits over the unified cell keys (the formula's P)
     passes = max(1, math.ceil(math.ceil(math.log2(max(cells, 2))) / 8))
     gb = grid_bytes(n_now, cells, passes)
     grid_gbs = gb / (ph["grid"] * 1e-3) / 1e9
+    gpath = grid_sort_path(c1["cap_low"], cells)
+    gpb = count_sort_bytes(n_now, cells) if gpath == "count" else gb
     traffic = _traffic(kernel_name)
     # every neighbour kernel and the grid build against its attainable
     # rate min(HBM peak x AI, FP32 peak), AI = algorithmic flops / bytes;
@@ -485,15 +508,20 @@ def main() -> int:
             "counter_bytes": tr["bytes"] if tr else None,
             "counter_gbs": round(tr["bytes"] / (ms * 1e-3) / 1e9, 1) if tr else None,
         }
-    # the grid build's counter bytes: hash, one histogram, scan and scatter
-    # per radix pass, reorder
-    gtr = {kn: _traffic(kn) for kn in ("k_begin_hash", "k_sort_hist", "k_sort_scan",
-                                       "k_sort_scatter", "k_reorder")}
+    # the grid build's counter bytes: the kernels of the sort path it takes
     grid_counter = None
-    if all(gtr.values()):
-        grid_counter = (gtr["k_begin_hash"]["bytes"] + gtr["k_reorder"]["bytes"] +
-                        passes * (gtr["k_sort_hist"]["bytes"] + gtr["k_sort_scan"]["bytes"] +
-                                  gtr["k_sort_scatter"]["bytes"]))
+    if gpath == "count":
+        gtr = {kn: _traffic(kn) for kn in ("k_begin_hash", "k_cell_sums", "k_cell_ranges",
+                                           "k_cell_scatter", "k_cell_reorder")}
+        if all(gtr.values()):
+            grid_counter = sum(v["bytes"] for v in gtr.values())
+    elif gpath == "radix":
+        gtr = {kn: _traffic(kn) for kn in ("k_begin_hash", "k_sort_hist", "k_sort_scan",
+                                           "k_sort_scatter", "k_reorder")}
+        if all(gtr.values()):
+            grid_counter = (gtr["k_begin_hash"]["bytes"] + gtr["k_reorder"]["bytes"] +
+                            passes * (gtr["k_sort_hist"]["bytes"] + gtr["k_sort_scan"]["bytes"] +
+                                      gtr["k_sort_scatter"]["bytes"]))
     roofline = {
         "kernel": kernel_name,
         "bound": "valu",
@@ -535,7 +563,9 @@ def main() -> int:
         "kernel_tflops": {k: round(v, 3) for k, v in tflops.items()},
         "roofline": roofline,
         "grid_roofline": {
-            "kernel": "grid build (hash + radix sort + ranges + reorder)",
+            "kernel": {"count": "grid build (hash + counting sort by cell + ranges + reorder)",
+                       "radix": "grid build (hash + radix sort + ranges + reorder)",
+                       "small": "grid build (hash + one-workgroup radix sort + reorder)"}[gpath],
             "bound": "hbm",
             "achieved": round(grid_gbs, 1),
             "peak": HBM_PEAK_GBS,
@@ -543,6 +573,9 @@ def main() -> int:
             "frac": round(grid_gbs / HBM_PEAK_GBS, 4),
             "algorithmic_bytes_per_launch": int(gb),
             "radix_passes": passes,
+            "sort_path": gpath,
+            "path_bytes_per_launch": int(gpb),
+            "path_frac": round(gpb / (ph["grid"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "counter_bytes": grid_counter,
             "counter_gbs": (round(grid_counter / (ph["grid"] * 1e-3) / 1e9, 1)
                             if grid_counter else None),
