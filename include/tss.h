@@ -282,7 +282,7 @@ void tss_destroy(tss_solver* s);
  * bytes such a batch holds. No counterpart in the reference, which allows
  * one solver per process (R/WCSPHSolver.cu:23-48,2728). */
 typedef struct tss_batch_desc {
-    int max_scenes;
+    int max_scenes;        /* 1 .. 65535                                */
     long long cap_low;     /* total low-particle capacity (children 4x) */
     long long cap_bnd;     /* total boundary particles                  */
     long long cells_low;   /* total low-grid cells                      */

@@ -184,7 +184,7 @@ struct DevState {
     float2* srp;              // [5 CL] cell-sorted (rho, p / rho^2)
     uint8_t* sstate;          // [5 CL + kSortedPad]
     uint8_t* ssurf;           // surface flag (sorted)
-    int* sscene;
+    uint16_t* sscene;         // [5 CL] scene of each sorted entry
     int* sidx;                // [5 CL] sorted index of unified id
     // [5 CL * kMaskWords] per cell-sorted target: the same-level
     // candidates its density pass accepted, one bit per candidate of its

@@ -264,7 +264,7 @@ struct tss_batch {
         b += (long long)tss::kRadix * tiles * 4 + tss::kRadix * 4;
         b += (ds.cells_low + ds.cells_high) * 8 + ds.cells_low * 9;
         b += ds.cap_bnd * 12;
-        b += U * (8 + 16 + 8 + 1 + 1 + 4 + 4);  // sorted copies + sidx
+        b += U * (8 + 16 + 8 + 1 + 1 + 2 + 4);  // sorted copies + sidx
         b += tss::kSortedPad * (8 + 1);
         b += U * 16;                               // sorted post-step state
         b += U * 4 * tss::kMaskWords;              // candidate masks
@@ -283,6 +283,8 @@ struct tss_batch {
     }
 
     void create(const tss_batch_desc& ds) {
+        // the cell-sorted scene index is 16-bit
+        require(ds.max_scenes <= 65535, "a batch holds at most 65535 scenes");
         require(ds.max_scenes >= 1 && ds.cap_low >= 1 && ds.cap_bnd >= 0 &&
                     ds.cells_low >= 1 && ds.cells_high >= 1,
                 "invalid batch capacities");
@@ -342,7 +344,7 @@ struct tss_batch {
         d.srp = dalloc<float2>(U, owned);
         d.sstate = dalloc<uint8_t>(U + tss::kSortedPad, owned);
         d.ssurf = dalloc<uint8_t>(U, owned);
-        d.sscene = dalloc<int>(U, owned);
+        d.sscene = dalloc<uint16_t>(U, owned);
         d.sidx = dalloc<int>(U, owned);
         d.nmask = dalloc<uint32_t>((size_t)U * tss::kMaskWords, owned);
         d.cand = dalloc<uint8_t>(CL, owned);

@@ -638,7 +638,7 @@ __global__ void k_cell_reorder(DevState d, const uint32_t* __restrict__ skeys,
         d.sstate[k] = sst;
         d.sidx[u] = k;
         const int s = d.cscene[key];
-        d.sscene[k] = s;
+        d.sscene[k] = (uint16_t)s;
         if (!(sst & 1u)) {
             if (k >= nl) d.scnt[s].prom = 1;
             else d.scnt[s].act = 1;
@@ -967,7 +967,7 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
         d.sidx[u] = k;
         // the scene owns the cell (cells of a scene are one range)
         const int s = d.cscene[key];
-        d.sscene[k] = s;
+        d.sscene[k] = (uint16_t)s;
         // scene segments per level: low part [0, nl), high part [nl, n)
         const bool high = k >= nl;
         // state-0 particles present: promoted children make the low
