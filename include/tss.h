@@ -482,7 +482,7 @@ typedef struct tss_serve_stats {
     double busy_s;                  /* time with at least one scene resident */
     long long particle_steps;
     double particle_steps_per_s;    /* particle_steps / wall_s            */
-    int ticks, graph_ticks;         /* batch advances; of them graph pairs */
+    int ticks, graph_ticks;         /* batch advances; of them graph-replayed */
     int max_resident;
     double step_ms_p50, step_ms_p99;        /* per step, host wall clock  */
     double request_ms_p50, request_ms_p99;  /* arrival to finish          */
@@ -497,8 +497,9 @@ typedef struct tss_serve_stats {
  * have arrived, in arrival order, whenever the batch has room and the sum of
  * the resident scenes' device bytes stays within budget_bytes (a request
  * that does not fit waits; later ones may pass it); advances every resident
- * scene together (one launch per phase; hipGraph pairs of steps while the
- * resident set is unchanged); and retires each scene the step it completes
+ * scene together (one launch per phase; while the resident set is
+ * unchanged, hipGraph pairs of steps, about 2 ms of them per tick); and
+ * retires each scene the step it completes
  * its steps, copying the requested fields out first. Results of a scene do
  * not depend on the others resident with it. Requests that can never fit
  * (or are invalid) get a nonzero status; the run continues. */

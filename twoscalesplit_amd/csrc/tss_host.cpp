@@ -2427,6 +2427,11 @@ int tss_serve(const tss_batch_desc* cap, long long budget, tss_request* req, int
         size_t next = 0;
         long long admitted = 0;
         int stable = 0;  // ticks since the resident set last changed
+        // a stable resident set advances several steps per tick (graph
+        // pairs, one host wait), about kTickMs of device time, so that
+        // arrivals wait at most about that long for admission
+        constexpr double kTickMs = 2.0;
+        double step_est_ms = 0.0;  // the last tick's time per step
         double busy = 0.0, last_finish = 0.0;
         const auto t0 = std::chrono::steady_clock::now();
         auto now = [&] {
@@ -2507,11 +2512,16 @@ int tss_serve(const tss_batch_desc* cap, long long budget, tss_request* req, int
             if (changed) stable = 0;
             int minleft = res[0].left;
             for (auto& r : res) minleft = std::min(minleft, r.left);
-            // graph pairs once the resident set has held for two ticks
-            const int k = (stable >= 2 && minleft >= 2) ? 2 : 1;
+            // graph pairs once the resident set has held for two ticks; no
+            // scene is advanced past its last step
+            int k = 1;
+            if (stable >= 2 && minleft >= 2) {
+                const int want = step_est_ms > 0.0 ? (int)(kTickMs / step_est_ms) : 2;
+                k = std::min(std::max(want, 2), minleft);
+            }
             const double a = now();
-            if (k == 2) {
-                b->step_graph(2);
+            if (k >= 2) {
+                b->step_graph(k);
                 ++st->graph_ticks;
             } else {
                 b->step_eager(1);
@@ -2519,7 +2529,8 @@ int tss_serve(const tss_batch_desc* cap, long long budget, tss_request* req, int
             HIP_CHECK(hipStreamSynchronize(b->stream));
             const double e = now();
             busy += e - a;
-            for (int i = 0; i < k; ++i) step_ms.push_back((e - a) * 1e3 / k);
+            step_est_ms = (e - a) * 1e3 / k;
+            for (int i = 0; i < k; ++i) step_ms.push_back(step_est_ms);
             ++st->ticks;
             ++stable;
             bool removed = false;

@@ -166,6 +166,7 @@ def run_rate(seed: int, rate: float, n: int, pool_low: int,
         "rate_per_s": rate, "requests": n, "served": ok,
         "particle_steps_per_s": round(st["particle_steps_per_s"], 1),
         "wall_s": round(st["wall_s"], 3),
+        "busy_s": round(st["busy_s"], 3),
         "step_ms": {"p50": round(st["step_ms_p50"], 4),
                     "p99": round(st["step_ms_p99"], 4)},
         "request_ms": {"p50": round(st["request_ms_p50"], 1),
