@@ -1648,8 +1648,41 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
     int* __restrict__ nlist_h, int* __restrict__ nlist_g) {
     __shared__ int wsum[2][kScanWaves];
     const int nl = d.counts->n_l, nh = d.counts->n_h, ng = d.counts->n_g;
+    const bool events = d.counts->events != 0;
     int seg, t;
     tile_seg(sh, blockIdx.x, seg, t);
+    if (!events) {
+        // every live entry is kept, in order, and none splits or is
+        // re-listed: the next lists are copies of the sorted ids and the
+        // groups, and each scene's list bounds are its sorted segments
+        if (blockIdx.x == 0) {
+            for (int s = threadIdx.x; s < d.S; s += kScanThreads) {
+                SceneCount& c = d.scnt[s];
+                c.kl0 = c.seg_l0;
+                c.kl1 = c.seg_l1;
+                c.kh0 = c.seg_h0 - nl;
+                c.kh1 = c.seg_h1 - nl;
+                c.sp0 = 0;
+                c.sp1 = 0;
+            }
+        }
+        const int nseg = seg == 0 ? nl : seg == 1 ? nh : ng;
+        const uint32_t* src = seg == 2 ? (const uint32_t*)list_g : vals + (seg == 1 ? nl : 0);
+        int* dst = seg == 0 ? nlist_l : seg == 1 ? nlist_h : nlist_g;
+        // all of the thread's entries in flight, then stored
+        uint32_t v[kScanItems];
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) {
+            const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
+            v[q] = i < nseg ? src[i] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) {
+            const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
+            if (i < nseg) dst[i] = (int)v[q];
+        }
+        return;
+    }
     // tiles past their segment's live entries write nothing: whole block
     if (t * kScanTile >= (seg == 0 ? nl : seg == 1 ? nh : ng)) return;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1660,7 +1693,6 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
     const int keepL = d.seg_base[3], keepH = d.seg_base[4],
               keepG = d.seg_base[5];
     const int nseg = seg == 0 ? nl : seg == 1 ? nh : ng;
-    const bool events = d.counts->events != 0;
     // places entry i at output positions g0 / g1 (flags fq)
     auto emit = [&](int i, int g0, int g1, const Flags& fq) {
         if (seg == 0 && i < nl) {
@@ -1717,18 +1749,6 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
             if (fq.p1) nlist_l[keepL + (g1 - d.seg_base[2])] = p;
         }
     };
-    if (!events) {
-        // every live entry is kept, in order, and none splits or is
-        // re-listed: an entry's position is the tile's base plus its offset
-        // in the tile (the entries before it are live too)
-#pragma unroll
-        for (int q = 0; q < kScanItems; ++q) {
-            const int off = q * kScanThreads + (int)threadIdx.x;
-            const int i = t * kScanTile + off;
-            emit(i, run0 + off, run1, Flags{i < nseg ? 1 : 0, 0});
-        }
-        return;
-    }
     for (int q = 0; q < kScanItems; ++q) {
         const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
         const Flags fq = entry_flags(d, vals, list_g, seg, i, nl, nh, ng);
