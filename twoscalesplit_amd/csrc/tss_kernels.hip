@@ -1348,13 +1348,14 @@ __device__ __forceinline__ float2 clamp_child(float2 ph, float2 pl, double rmax)
 // group sit in one wave and OR their results in a ballot, and lane 0
 // applies the decision. LANES = 1 tests the children in turn.
 template <int LANES>
-__global__ void k_groups(DevState d, const int* __restrict__ list_g) {
+__global__ void k_groups(DevState d, const int* __restrict__ list_g, int solo) {
     static_assert(LANES == 1 || LANES == 4, "one lane or one per child");
     const int ng = d.counts->n_g;
     GRID_STRIDE(i, LANES * ng) {
         const int q = i / LANES, kl = i % LANES;
         const int p = list_g[q];
-        const int s = d.scene_of[p];
+        // with one resident scene (solo >= 0), no lookup by id
+        const int s = solo >= 0 ? solo : d.scene_of[p];
         const SceneDev& sc = d.scenes[s];
         const int c0 = d.CL + 4 * p;
         const uint8_t st = d.state[p];
@@ -1404,8 +1405,18 @@ __global__ void k_groups(DevState d, const int* __restrict__ list_g) {
             } else {
                 const float2 pl = d.pos[p];
                 const double rmax = clamp_radius(sc.m_l, d.rho[p]);
-                for (int k = 0; k < 4; ++k)
-                    d.pos[c0 + k] = clamp_child(d.pos[c0 + k], pl, rmax);
+                float2 ph[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) ph[k] = d.pos[c0 + k];
+                // stored only where the clamp changes a bit (within the
+                // radius it returns the position itself, as a rule)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float2 c = clamp_child(ph[k], pl, rmax);
+                    if (__float_as_uint(c.x) != __float_as_uint(ph[k].x) ||
+                        __float_as_uint(c.y) != __float_as_uint(ph[k].y))
+                        d.pos[c0 + k] = c;
+                }
             }
         } else if (st == kDeleted && sc.mode == 2 && !keep) {
             merged = 1;
@@ -2355,9 +2366,11 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     // few groups and long band tests: a lane per child, small blocks to
     // spread them over the CUs
     if (sh.band_any)
-        k_groups<4><<<blocks_for(4 * sh.cap_alive, 64, 16384), 64, 0, st>>>(d, d.list_g[par]);
+        k_groups<4><<<blocks_for(4 * sh.cap_alive, 64, 16384), 64, 0, st>>>(d, d.list_g[par],
+                                                                           sh.solo);
     else
-        k_groups<1><<<blocks_for(sh.cap_alive, 256, 4096), 256, 0, st>>>(d, d.list_g[par]);
+        k_groups<1><<<blocks_for(sh.cap_alive, 256, 4096), 256, 0, st>>>(d, d.list_g[par],
+                                                                         sh.solo);
     // the cap can bind only where cap_high < 4 cap_low: a scene's groups
     // and candidates never exceed its low ids
     if (sh.cap_binds) k_cap<<<sh.S, 256, 0, st>>>(d);
