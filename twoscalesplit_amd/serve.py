@@ -157,13 +157,14 @@ def scene_at_a_time(trace: Trace, measured=None) -> dict:
 
 def run_rate(seed: int, rate: float, n: int, pool_low: int,
              baseline: bool = True, budget: int = BUDGET_BYTES,
-             scene_list=None, measured=None) -> dict:
+             scene_list=None, measured=None, device_gen: bool = False) -> dict:
     tr = Trace(seed, rate, n, scene_list=scene_list)
     desc = pool_desc(tr, pool_low)
-    st, per, _ = serve(tr, desc, budget)
+    st, per, _ = serve(tr, desc, budget, device_gen=device_gen)
     ok = sum(1 for p in per if p["status"] == 0)
     out = {
         "rate_per_s": rate, "requests": n, "served": ok,
+        "admission": "device-generated fluid blocks" if device_gen else "uploaded positions",
         "particle_steps_per_s": round(st["particle_steps_per_s"], 1),
         "wall_s": round(st["wall_s"], 3),
         "busy_s": round(st["busy_s"], 3),
@@ -198,6 +199,9 @@ def main(argv=None) -> int:
                     help="low-particle capacity of the serving pool")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-baseline", action="store_true")
+    ap.add_argument("--device-gen", action="store_true",
+                    help="requests carry generator specs; fluid blocks are "
+                         "generated on the device at admission")
     args = ap.parse_args(argv)
     scene_list = Trace(args.seed, 1.0, args.requests).scenes
     measured = None
@@ -210,7 +214,7 @@ def main(argv=None) -> int:
               flush=True)
         res = run_rate(args.seed, r, args.requests, args.pool_low,
                        not args.no_baseline, int(args.budget_gb * 1e9),
-                       scene_list, measured)
+                       scene_list, measured, args.device_gen)
         print(json.dumps(res), flush=True)
     return 0
 
