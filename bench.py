@@ -479,7 +479,8 @@ def main() -> int:
     traffic = _traffic(kernel_name)
     # every neighbour kernel and the grid build against its attainable
     # rate min(HBM peak x AI, FP32 peak), AI = algorithmic flops / bytes;
-    # bytes per level (SURVEY's compulsory-traffic formulas): density
+    # bytes per level (compulsory traffic: each particle's own data once,
+    # the cell table, the other level's data once): density
     # N 17 + 8 C + N_other 9, force + integration N 49 + 8 C + N_other 25
     # + N_B 8; counter bytes from the committed PMC summary
     nl_, nh_ = c1["n_low_last"], c1["n_high_last"]
