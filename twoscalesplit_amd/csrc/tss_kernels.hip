@@ -196,6 +196,16 @@ __device__ __forceinline__ float gbar_half(const SceneDev& sc, float r) {
     return sc.gc_h2 * (q < 0.5f ? ga : gb);
 }
 
+// gbar_half without its coefficient gc_h2 (the force passes fold it into
+// the pair's mass factor).
+__device__ __forceinline__ float gbar_half_unit(const SceneDev& sc, float r) {
+    const float q = r * sc.q_h;
+    const float u = 1.0f - q;
+    const float ga = q * fmaf(-99.0f, q, 68.0f);
+    const float gb = fmaf(u, fmaf(29.0f, u, 2.0f), 1.0f);
+    return q < 0.5f ? ga : gb;
+}
+
 // Kernel forms, chosen per scene: KM 0 m4_blend, 1 m4_blend_inner (hh_inner),
 // 2 wbar_half / gbar_half (hh_half).
 template <int KM>
