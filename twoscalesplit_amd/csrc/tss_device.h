@@ -317,7 +317,10 @@ void launch_copy16(const void* src, void* dst, long long n16, void* stream);
 // n 8-byte reads, one every `stride` 8-byte words of src (src holds at least
 // n * stride words); sink: one 8-byte word.
 void launch_gather_probe(const void* src, long long n, int stride, void* sink, void* stream);
+// Stable removal of one scene's entries from the current lists (a scene
+// retired between steps), into the other list buffers: the caller flips
+// the list parity afterwards. n_l, n_h, n_g: the list sizes before.
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
-                         void* stream);
+                         int n_l, int n_h, int n_g, void* stream);
 
 }  // namespace tss

@@ -823,7 +823,11 @@ struct tss_batch {
     void remove_scene(int sid) {
         require(sid >= 0 && sid < max_scenes && scenes[sid].alive, "no such scene");
         SceneHost& s = scenes[sid];
-        tss::launch_filter_scene(d, sh, sid, stream);
+        {
+            const tss::DevCounts c = read_counts();
+            tss::launch_filter_scene(d, sh, sid, c.n_l, c.n_h, c.n_g, stream);
+            sh.list_parity ^= 1;  // the filtered lists are in the other buffers
+        }
         tss::SceneDev sd{};
         HIP_CHECK(hipMemcpyAsync(d.scenes + sid, &sd, sizeof(sd), hipMemcpyHostToDevice, stream));
         tss::launch_fill_int(d.bstart + s.cbl, -1, s.ncl, stream);

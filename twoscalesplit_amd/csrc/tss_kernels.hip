@@ -2022,58 +2022,142 @@ __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
     }
 }
 
-// Stable removal of one scene's entries from the current lists (used when a
-// scene is retired between steps). Single block, serial-in-tile order.
-__global__ __launch_bounds__(1024) void k_filter_lists(DevState d, int scene,
-                                                      int* __restrict__ ll,
-                                                      int* __restrict__ lh,
-                                                      int* __restrict__ lg) {
+// Removal of a scene's entries from the lists: a stable filter from the
+// current list buffers into the other ones, in three launches over the
+// compaction's tiles (low, high, group segments): kept entries per tile
+// (and, in the high segment, kept entries of its fresh tail), one block's
+// prefix over the tiles with the new counts, then the scatter (each entry
+// ranked by wave ballots within its tile, in index order).
+__device__ __forceinline__ bool filter_keep(const DevState& d, int seg, int v, int scene) {
+    const int lid = seg == 1 ? (v - d.CL) >> 2 : v;
+    return d.scene_of[lid] != scene;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_filter_count(
+    DevState d, StepShape sh, int scene, const int* __restrict__ ll,
+    const int* __restrict__ lh, const int* __restrict__ lg) {
+    __shared__ int wsum[2][kScanWaves];
+    int seg, t;
+    tile_seg(sh, blockIdx.x, seg, t);
+    const int n = seg == 0 ? d.counts->n_l : seg == 1 ? d.counts->n_h : d.counts->n_g;
+    const int tail0 = seg == 1 ? n - d.counts->n_fresh : n;
+    const int* L = seg == 0 ? ll : seg == 1 ? lh : lg;
+    int a = 0, f = 0;  // wave totals: kept entries, kept in the fresh tail
+    for (int q = 0; q < kScanItems; ++q) {
+        const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
+        const bool keep = i < n && filter_keep(d, seg, L[i], scene);
+        a += __popcll(__ballot(keep));
+        f += __popcll(__ballot(keep && i >= tail0));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        wsum[0][w] = a;
+        wsum[1][w] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ta = 0, tf = 0;
+        for (int v = 0; v < kScanWaves; ++v) {
+            ta += wsum[0][v];
+            tf += wsum[1][v];
+        }
+        d.tile_sums[2 * blockIdx.x] = ta;
+        d.tile_sums[2 * blockIdx.x + 1] = tf;
+    }
+}
+
+// One block: per segment, exclusive prefix of the tiles' kept counts (the
+// tiles' output bases) and the segment's new size; the high segment's kept
+// fresh entries give the new n_fresh.
+__global__ __launch_bounds__(1024) void k_filter_top(DevState d, StepShape sh) {
     __shared__ int part[1024];
-    __shared__ int base;
-    __shared__ int fresh;  // kept entries of the high list's fresh tail
-    int* lists[3] = {ll, lh, lg};
-    int* ns[3] = {&d.counts->n_l, &d.counts->n_h, &d.counts->n_g};
-    for (int which = 0; which < 3; ++which) {
-        int* L = lists[which];
-        const int n = *ns[which];
-        const int tail0 = which == 1 ? n - d.counts->n_fresh : n;
-        if (threadIdx.x == 0) {
-            base = 0;
-            fresh = 0;
+    const int start[3] = {0, sh.scan_tiles_l, sh.scan_tiles_l + sh.scan_tiles_h};
+    const int len[3] = {sh.scan_tiles_l, sh.scan_tiles_h, sh.scan_tiles_g};
+    int fresh = 0;
+    for (int sg = 0; sg < 3; ++sg) {
+        const int nt = len[sg];
+        const int per = (nt + 1023) / 1024;
+        const int t0 = (int)threadIdx.x * per;
+        int a = 0, f = 0;
+        for (int i = 0; i < per; ++i) {
+            if (t0 + i < nt) {
+                a += d.tile_sums[2 * (start[sg] + t0 + i)];
+                f += d.tile_sums[2 * (start[sg] + t0 + i) + 1];
+            }
+        }
+        part[threadIdx.x] = a;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += v;
+            __syncthreads();
+        }
+        int run = part[threadIdx.x] - a;
+        const int tot = part[1023];
+        for (int i = 0; i < per; ++i) {
+            if (t0 + i < nt) {
+                const int tt = start[sg] + t0 + i;
+                const int v = d.tile_sums[2 * tt];
+                d.tile_base[2 * tt] = run;
+                run += v;
+            }
+        }
+        // the fresh tail's kept entries (high segment only)
+        if (sg == 1) {
+            __syncthreads();
+            part[threadIdx.x] = f;
+            __syncthreads();
+            for (int off = 512; off > 0; off >>= 1) {
+                if (threadIdx.x < (unsigned)off) part[threadIdx.x] += part[threadIdx.x + off];
+                __syncthreads();
+            }
+            fresh = part[0];
         }
         __syncthreads();
-        for (int b0 = 0; b0 < n; b0 += 1024) {
-            int i = b0 + threadIdx.x;
-            int v = 0, keep = 0;
-            if (i < n) {
-                v = L[i];
-                int lid = which == 0 ? v : (which == 1 ? (v - d.CL) >> 2 : v);
-                keep = d.scene_of[lid] != scene;
-            }
-            part[threadIdx.x] = keep;
-            __syncthreads();
-            for (int off = 1; off < 1024; off <<= 1) {
-                int a = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
-                __syncthreads();
-                part[threadIdx.x] += a;
-                __syncthreads();
-            }
-            int pos = base + part[threadIdx.x] - keep;
-            int tot = part[1023];
-            if (keep && i >= tail0) atomicAdd(&fresh, 1);
-            __syncthreads();
-            // in-place is safe: pos <= i and every read of this chunk
-            // happened before the barriers above
-            if (keep) L[pos] = v;
-            __syncthreads();
-            if (threadIdx.x == 0) base += tot;
-            __syncthreads();
-        }
         if (threadIdx.x == 0) {
-            *ns[which] = base;
-            if (which == 1) d.counts->n_fresh = fresh;
+            if (sg == 0) d.counts->n_l = tot;
+            if (sg == 1) {
+                d.counts->n_h = tot;
+                d.counts->n_fresh = fresh;
+            }
+            if (sg == 2) d.counts->n_g = tot;
         }
         __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_filter_scatter(
+    DevState d, StepShape sh, int scene, const int* __restrict__ ll,
+    const int* __restrict__ lh, const int* __restrict__ lg, int* __restrict__ nll,
+    int* __restrict__ nlh, int* __restrict__ nlg, int n_l, int n_h, int n_g) {
+    __shared__ int wsum[kScanWaves];
+    int seg, t;
+    tile_seg(sh, blockIdx.x, seg, t);
+    // the segment sizes before the filter (k_filter_top has rewritten the
+    // counts): passed by the host, which read them before the launch
+    const int n = seg == 0 ? n_l : seg == 1 ? n_h : n_g;
+    if (t * kScanTile >= n) return;  // whole block
+    const int* L = seg == 0 ? ll : seg == 1 ? lh : lg;
+    int* O = seg == 0 ? nll : seg == 1 ? nlh : nlg;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int run = d.tile_base[2 * blockIdx.x];
+    for (int q = 0; q < kScanItems; ++q) {
+        const int i = t * kScanTile + q * kScanThreads + (int)threadIdx.x;
+        const int v = i < n ? L[i] : 0;
+        const bool keep = i < n && filter_keep(d, seg, v, scene);
+        const uint64_t m = __ballot(keep);
+        if (lane == 0) wsum[wv] = __popcll(m);
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int w = 0; w < kScanWaves; ++w) {
+            pre += w < wv ? wsum[w] : 0;
+            tot += wsum[w];
+        }
+        __syncthreads();  // wsum is rewritten by the next chunk
+        if (keep) O[run + pre + __popcll(m & lt)] = v;
+        run += tot;
     }
 }
 
@@ -2409,10 +2493,16 @@ void launch_frame_pack(const DevState& d, const int* list, int n, int lo, int hi
 int frame_tiles(int n) { return (n + kFrameTile - 1) / kFrameTile; }
 
 void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
-                         void* stream) {
+                         int n_l, int n_h, int n_g, void* stream) {
+    hipStream_t st = (hipStream_t)stream;
     const int par = sh.list_parity;
-    k_filter_lists<<<1, 1024, 0, (hipStream_t)stream>>>(
-        d, scene, d.list_l[par], d.list_h[par], d.list_g[par]);
+    const int ntiles = sh.scan_tiles_l + sh.scan_tiles_h + sh.scan_tiles_g;
+    k_filter_count<<<ntiles, kScanThreads, 0, st>>>(d, sh, scene, d.list_l[par],
+                                                     d.list_h[par], d.list_g[par]);
+    k_filter_top<<<1, 1024, 0, st>>>(d, sh);
+    k_filter_scatter<<<ntiles, kScanThreads, 0, st>>>(
+        d, sh, scene, d.list_l[par], d.list_h[par], d.list_g[par], d.list_l[par ^ 1],
+        d.list_h[par ^ 1], d.list_g[par ^ 1], n_l, n_h, n_g);
 }
 
 void launch_work_counts(const DevState& d, const StepShape& sh,
