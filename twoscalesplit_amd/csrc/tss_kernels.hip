@@ -1172,27 +1172,44 @@ __global__ __launch_bounds__(256) void k_reduce(DevState d,
         const long long i0 = (long long)chunk * kRedChunk;
         const long long i1 = i0 + kRedChunk < m ? i0 + kRedChunk : (long long)m;
         const double g = (double)sc.g;
-        for (long long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            const bool high = i >= nl_s;
-            int k = !high ? c.seg_l0 + (int)i : c.seg_h0 + (int)(i - nl_s);
-            // post-step position and velocity (the force pass's sorted copy)
-            const float4 xv = d.snew[k];
-            double vx = xv.z, vy = xv.w;
-            double v2 = vx * vx + vy * vy;
-            acc[5] = fmax(acc[5], v2);
-            if (!high || d.sstate[k] == 0) {
-                double mm = high ? (double)sc.m_h : (double)sc.m_l;
-                double y = (double)xv.y;
-                double ke = 0.5 * (mm * v2);
-                double pe = (mm * g) * y;
-                acc[0] += mm;
-                acc[1] += mm * vx;
-                acc[2] += mm * vy;
-                acc[3] += ke;
-                acc[4] += pe;
-                acc[6] += fabs(mm * vx);
-                acc[7] += fabs(mm * vy);
-                acc[8] += fabs(pe);
+        // four entries per thread and trip, their loads in flight together
+        // (one load per trip left the loop waiting on each in turn); added
+        // in the same order as one entry per trip
+        const long long bd = blockDim.x;
+        for (long long i = i0 + threadIdx.x; i < i1; i += 4 * bd) {
+            float4 xv[4];
+            uint8_t st[4];
+            bool hi[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const long long ii = i + q * bd < i1 ? i + q * bd : i;
+                hi[q] = ii >= nl_s;
+                const int k = !hi[q] ? c.seg_l0 + (int)ii : c.seg_h0 + (int)(ii - nl_s);
+                // post-step position and velocity (the force pass's sorted copy)
+                xv[q] = d.snew[k];
+                st[q] = d.sstate[k];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (i + q * bd >= i1) break;
+                const bool high = hi[q];
+                double vx = xv[q].z, vy = xv[q].w;
+                double v2 = vx * vx + vy * vy;
+                acc[5] = fmax(acc[5], v2);
+                if (!high || st[q] == 0) {
+                    double mm = high ? (double)sc.m_h : (double)sc.m_l;
+                    double y = (double)xv[q].y;
+                    double ke = 0.5 * (mm * v2);
+                    double pe = (mm * g) * y;
+                    acc[0] += mm;
+                    acc[1] += mm * vx;
+                    acc[2] += mm * vy;
+                    acc[3] += ke;
+                    acc[4] += pe;
+                    acc[6] += fabs(mm * vx);
+                    acc[7] += fabs(mm * vy);
+                    acc[8] += fabs(pe);
+                }
             }
         }
     }
