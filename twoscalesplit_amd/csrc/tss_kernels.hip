@@ -1828,11 +1828,29 @@ __global__ __launch_bounds__(256) void k_finalize(DevState d) {
     const int nslots = d.red_n[s];
     if (sc.alive) {
         double part[kRedFields] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int ch = threadIdx.x; ch < nslots; ch += 256) {
-            const double* r = d.red + ((size_t)d.red_base[s] + ch) * kRedFields;
+        // two slots per trip, their loads in flight together; added in
+        // slot order as with one slot per trip
+        for (int ch = threadIdx.x; ch < nslots; ch += 512) {
+            const bool two = ch + 256 < nslots;
+            const double* r0 = d.red + ((size_t)d.red_base[s] + ch) * kRedFields;
+            const double* r1 = two ? r0 + 256 * kRedFields : r0;
+            double v0[kRedFields], v1[kRedFields];
+#pragma unroll
             for (int f = 0; f < kRedFields; ++f) {
-                if (f == 5) part[f] = fmax(part[f], r[f]);
-                else part[f] += r[f];
+                v0[f] = r0[f];
+                v1[f] = r1[f];
+            }
+#pragma unroll
+            for (int f = 0; f < kRedFields; ++f) {
+                if (f == 5) part[f] = fmax(part[f], v0[f]);
+                else part[f] += v0[f];
+            }
+            if (two) {
+#pragma unroll
+                for (int f = 0; f < kRedFields; ++f) {
+                    if (f == 5) part[f] = fmax(part[f], v1[f]);
+                    else part[f] += v1[f];
+                }
             }
         }
         for (int f = 0; f < kRedFields; ++f) sh[f][threadIdx.x] = part[f];
