@@ -1156,8 +1156,7 @@ __global__ void k_relax_commit(DevState d, const int* __restrict__ lh, int it) {
 // low, promoted high) and the max |v|^2 for the CFL time step. Fixed chunk
 // and tree shape -> bitwise reproducible, independent of other scenes.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_reduce(DevState d,
-                                                const uint32_t* __restrict__ vals) {
+__global__ __launch_bounds__(256) void k_reduce(DevState d) {
     __shared__ double sh[kRedFields][256];
     const int slot = blockIdx.x;
     const int s = d.red_scene[slot];
@@ -1730,7 +1729,6 @@ __global__ __launch_bounds__(kScanThreads) void k_compact_scatter(
     int run1 = d.tile_base[2 * blockIdx.x + 1];
     const int keepL = d.seg_base[3], keepH = d.seg_base[4],
               keepG = d.seg_base[5];
-    const int nseg = seg == 0 ? nl : seg == 1 ? nh : ng;
     // places entry i at output positions g0 / g1 (flags fq)
     auto emit = [&](int i, int g0, int g1, const Flags& fq) {
         if (seg == 0 && i < nl) {
@@ -2489,7 +2487,7 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     mark(4);
     launch_pair<true, true>(d, vals, keys, sh.tune[kTuneForceHigh], 4 * sh.cap_alive, st);
     mark(5);
-    k_reduce<<<sh.red_slots, 256, 0, st>>>(d, vals);
+    k_reduce<<<sh.red_slots, 256, 0, st>>>(d);
     mark(6);
     k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
     // few groups and long band tests: a lane per child, small blocks to
