@@ -110,7 +110,7 @@ struct SceneCount {
     int kl0, kl1, kh0, kh1, sp0, sp1;
     int prom;                 // 1: a listed high particle is promoted (state 0)
     int act;                  // 1: a listed low particle is active (state 0)
-    int pad1;
+    int hsurf;                // 1: a listed high particle is a free-surface particle
     unsigned long long psteps;  // particle-steps advanced since admission
 };
 

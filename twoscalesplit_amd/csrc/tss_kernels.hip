@@ -353,6 +353,7 @@ __global__ void k_begin_hash(DevState d, int nclear, int count, int solo,
             c.kl0 = c.kl1 = c.kh0 = c.kh1 = c.sp0 = c.sp1 = 0;
             c.prom = 0;
             c.act = 0;
+            c.hsurf = 0;
         }
         return;
     }
@@ -1258,9 +1259,10 @@ __device__ __forceinline__ uint32_t flag_bits8(const uint8_t* __restrict__ f,
 // Any surface particle within sqrt-distance r of x (r2 <= t)? Scans the
 // low box and the high box of radius r on the cell-sorted pre-step state.
 // Only cells flagged as holding a surface particle (csurf, set by the
-// density pass) are read.
+// density pass) are read, and the high box only where the scene has a high
+// surface particle this step (hsurf).
 __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
-                        float r, float t) {
+                        float r, float t, bool hsurf) {
     // coarse test first: a surface particle within r of x lies in a low cell
     // at most one cell outside x's low box of radius r (rounding included)
     // and marks its own low cell and the eight around it (cband), so a box
@@ -1277,7 +1279,8 @@ __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
         }
         if (!any) return false;
     }
-    for (int lv = 0; lv < 2; ++lv) {
+    // the high cells only where the scene has a high surface particle
+    for (int lv = 0; lv < (hsurf ? 2 : 1); ++lv) {
         const bool high = lv == 1;
         const Box b = scan_box(sc, high, x, r);
         const int base = high ? sc.cbase_h : sc.cbase_l;
@@ -1322,7 +1325,7 @@ __global__ void k_split_cand(DevState d, const uint32_t* __restrict__ vals) {
             if (sc.mode == 1) {
                 cand = d.pos[vals[k]].x > sc.split_x;  // predicted x' (R:1835)
             } else {
-                cand = in_band(d, sc, d.spos[k], sc.r_band, sc.t_band);
+                cand = in_band(d, sc, d.spos[k], sc.r_band, sc.t_band, d.scnt[s].hsurf != 0);
             }
         }
         d.cand[k] = cand;
@@ -1402,10 +1405,12 @@ __global__ void k_groups(DevState d, const int* __restrict__ list_g, int solo) {
         bool keep = false;
         if (LANES == 1) {
             for (int k = 0; k < 4 && test && !keep; ++k)
-                keep = in_band(d, sc, d.spos[d.sidx[c0 + k]], sc.r_merge, sc.t_merge);
+                keep = in_band(d, sc, d.spos[d.sidx[c0 + k]], sc.r_merge, sc.t_merge,
+                               d.scnt[s].hsurf != 0);
         } else {
             const bool mine = test && in_band(d, sc, d.spos[d.sidx[c0 + kl]],
-                                              sc.r_merge, sc.t_merge);
+                                              sc.r_merge, sc.t_merge,
+                                              d.scnt[s].hsurf != 0);
             const uint64_t bal = __ballot(mine);
             const int g0 = (threadIdx.x & 63) & ~(LANES - 1);
             keep = ((bal >> g0) & ((1ull << LANES) - 1ull)) != 0ull;
