@@ -1068,12 +1068,14 @@ struct tss_batch {
         set_timing(false);
         step_eager(1);
         taken += 1;
-        // stage 1
+        // stage 1: block and depth, with the XCD order (the usual winner on
+        // dense populations, where the block size that suits it can differ
+        // from the one that suits round-robin blocks)
         std::vector<std::array<tss::PairTune, tss::kPairKernels>> cand;
         for (int b : {128, 256, 512})
             for (int dp : {8, 16, 32}) {
                 std::array<tss::PairTune, tss::kPairKernels> a;
-                for (auto& t : a) t = {b, dp, 0, 0};
+                for (auto& t : a) t = {b, dp, 1, 0};
                 cand.push_back(a);
             }
         std::vector<std::array<double, tss::kPairKernels>> tot;
@@ -1087,11 +1089,11 @@ struct tss_batch {
             best[k] = cand[bi][k];
             best_t[k] = tot[bi][k];
         }
-        // stage 2 (the plain-grid, plain-order variant is stage 1's winner)
+        // stage 2 (the plain-grid, XCD-order variant is stage 1's winner)
         cand.clear();
         for (int swz : {0, 1})
             for (int gcap : {0, 4 * cus, 8 * cus}) {
-                if (swz == 0 && gcap == 0) continue;
+                if (swz == 1 && gcap == 0) continue;
                 std::array<tss::PairTune, tss::kPairKernels> a;
                 for (int k = 0; k < tss::kPairKernels; ++k) a[k] = {best[k].block, best[k].depth, swz, gcap};
                 cand.push_back(a);
