@@ -455,7 +455,7 @@ def main() -> int:
     fl = {
         "density_low": W_CAND * (work["cand_ll"] + work["cand_lh"]) +
                        W_DENS * work["dens_pairs_l"],
-        "density_high": W_CAND * (work["cand_hh"] + work["cand_hl"]) +
+        "density_high": W_CAND * (work["cand_hh"] + work["cand_hl_density"]) +
                         W_DENS * work["dens_pairs_h"],
         "force_low": W_CAND * (work["dens_pairs_l"] + work["cand_lh"]) +
                      W_FORCE * work["force_pairs_l"] + W_WALL * work["wall_l"],

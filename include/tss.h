@@ -231,11 +231,13 @@ int tss_enable_timing(tss_solver* s, int on);
 int tss_phase_times(tss_solver* s, float out_ms[TSS_PHASE_COUNT]);
 /* Exact work of the density and force kernels on the current state (grid
  * rebuilt from the current lists), per target level: candidates [0] low
- * same-level [1] low cross-level [2] high same-level [3] high cross-level;
- * density pairs [4] low [5] high; force pairs [6] low [7] high; wall
- * candidates [8] low [9] high; targets [10] low [11] high. Multiplied by
- * fixed per-item flop weights these give each kernel's algorithmic flops. */
-#define TSS_WORK_COUNT 12
+ * same-level [1] low cross-level [2] high same-level [3] high cross-level
+ * (the force's radius-h box); density pairs [4] low [5] high; force pairs
+ * [6] low [7] high; wall candidates [8] low [9] high; targets [10] low [11]
+ * high; [12] high cross-level candidates of the density (its radius-H box).
+ * Multiplied by fixed per-item flop weights these give each kernel's
+ * algorithmic flops. */
+#define TSS_WORK_COUNT 13
 int tss_work_counts(tss_solver* s, unsigned long long out[TSS_WORK_COUNT]);
 /* Launch shapes of the four neighbour kernels, index order: [0] density low
  * targets, [1] density high, [2] force low, [3] force high. block: 128, 256

@@ -136,6 +136,40 @@ def test_promote_merge_dissolve_parity():
     assert ev[0] > 0 and ev[1] > 0, ev
 
 
+def test_work_counts_match_the_kernels_pairs():
+    # tss_work_counts (the flop counts of the roofline) against the pairs the
+    # density and force kernels themselves add (tss_pair_sets, their
+    # recording variant) on the same states: density pairs (same + cross
+    # level), force pairs and wall candidates per level, targets; the
+    # density's radius-H cross-level box scans no more candidates than the
+    # force's radius-h box and at least the pairs it adds
+    sc = scenes.small_two_scale(nx=24, ny=32, mode=MODE_BAND)
+    sc.cfg.promote_min_age = 1
+    sc.cfg.promote_max_age = 3
+    sc.cfg.promote_fluc = 1.0
+    sol = _solver(sc)
+    try:
+        seen_hl = 0
+        for adv in (1, 3, 4):
+            sol.step(adv)
+            _say(sc.name, "work counts at step", sol.counts()["step"])
+            w = sol.work_counts()
+            cl, _ = sol.pair_sets(0)
+            ch, _ = sol.pair_sets(1)
+            assert w["dens_pairs_l"] == int(cl[:, 0].sum()) + int(cl[:, 1].sum())
+            assert w["dens_pairs_h"] == int(ch[:, 0].sum()) + int(ch[:, 1].sum())
+            assert w["force_pairs_l"] == int(cl[:, 2].sum()) + int(cl[:, 3].sum())
+            assert w["force_pairs_h"] == int(ch[:, 2].sum()) + int(ch[:, 3].sum())
+            assert w["wall_l"] == int(cl[:, 4].sum())
+            assert w["wall_h"] == int(ch[:, 4].sum())
+            assert w["targets_l"] == len(cl) and w["targets_h"] == len(ch)
+            assert int(ch[:, 1].sum()) <= w["cand_hl_density"] <= w["cand_hl"]
+            seen_hl += int(ch[:, 1].sum())
+        assert seen_hl > 0, "no cross-level density pair of a high target"
+    finally:
+        sol.close()
+
+
 def test_wide_high_radius_parity():
     # H = 0.6 h: a high target's own pairs reach past 2r/h = 1, so the high
     # passes take the general blended kernel (H = h/2 scenes take W_h's

@@ -262,7 +262,7 @@ constexpr int kTimingPoints = 9;
 // Work counters of k_work: candidates (low same, low cross, high same, high
 // cross), density pairs (low, high), force pairs (low, high), wall
 // candidates (low, high), targets (low, high)
-constexpr int kWorkCounts = 12;
+constexpr int kWorkCounts = 13;
 
 // Kernel launchers (tss_kernels.hip). All enqueue on `stream`.
 // timing_events: hipEvent_t[kTimingPoints] or nullptr; event_flags is passed

@@ -2026,6 +2026,18 @@ __global__ void k_work(DevState d, unsigned long long* __restrict__ out) {
         }
         // cross level
         if (high || d.scnt[s].seg_h1 > d.scnt[s].seg_h0) {
+            // the density's high <- low scan: the low cells of the radius-H
+            // box (its cutoff), the same rows' state filter
+            if (high) {
+                const Box bd = scan_box(sc, false, xi, sc.H);
+                const uint8_t need = i_prom ? 0u : 1u;
+                for (int cy = bd.y0; cy <= bd.y1; ++cy) {
+                    int st, en;
+                    row_run_need(d.cstart, d.cend, d.cflag, sc.cbase_l + cy * sc.dlx, bd.x0,
+                                 bd.x1, need, st, en);
+                    if (en > st) w[12] += (unsigned long long)(en - st);
+                }
+            }
             Box b = scan_box(sc, !high, xi, sc.h);
             const uint8_t need = high ? (i_prom ? 0u : 1u) : 1u;
             for (int cy = b.y0; cy <= b.y1; ++cy) {

@@ -25,7 +25,7 @@ PHASES = ["grid", "density_low", "density_high", "force_low", "force_high",
 # tss_work_counts order
 WORK = ["cand_ll", "cand_lh", "cand_hh", "cand_hl", "dens_pairs_l",
         "dens_pairs_h", "force_pairs_l", "force_pairs_h", "wall_l", "wall_h",
-        "targets_l", "targets_h"]
+        "targets_l", "targets_h", "cand_hl_density"]
 
 
 def _field_shape(name: str, cnt: dict) -> tuple[np.dtype, tuple]:
