@@ -313,9 +313,10 @@ int tss_batch_get_tuning(tss_batch* b, tss_tuning* out);
 int tss_batch_set_tuning(tss_batch* b, const tss_tuning* t);
 /* Autotuner: times candidate shapes with hipEvents while advancing the
  * batch by real steps (they count, the results being shape-independent).
- * After one ordinary step, stage 1 tries every block x depth, stage 2 the
- * XCD order and persistent grid caps (4 and 8 blocks per CU) on each
- * kernel's stage-1 winner; each candidate runs `rounds` times for
+ * After one ordinary step, stage 1 tries every block x depth under the XCD
+ * order, stage 2 round-robin order and persistent grid caps (4 and 8
+ * blocks per CU) on each kernel's stage-1 winner; each candidate runs
+ * `rounds` times for
  * steps_per_trial steps, in alternating order; a candidate's time per
  * target of the kernel's level is its fastest trial, and per kernel the
  * fastest candidate is kept (and returned in out).
