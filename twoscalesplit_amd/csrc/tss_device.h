@@ -185,7 +185,8 @@ struct DevState {
     uint8_t* sstate;          // [5 CL + kSortedPad]
     uint8_t* ssurf;           // surface flag (sorted)
     uint16_t* sscene;         // [5 CL] scene of each sorted entry
-    int* sidx;                // [5 CL] sorted index of unified id
+    int* sidx;                // [5 CL] sorted index of unified id (written
+                              // where read: band rule, relaxation, inspection)
     // [5 CL * kMaskWords] per cell-sorted target: the same-level
     // candidates its density pass accepted, one bit per candidate of its
     // scan rows (each row starting on a word), read by the force pass

@@ -579,7 +579,7 @@ __global__ void k_cell_scatter(DevState d, const uint32_t* __restrict__ keys,
 // cell state flags, scene flags. The id's data is loaded before the rank is
 // formed, so its latency overlaps the ranking.
 __global__ void k_cell_reorder(DevState d, const uint32_t* __restrict__ skeys,
-                               uint32_t* __restrict__ svals) {
+                               uint32_t* __restrict__ svals, int with_sidx) {
     const int nl = d.counts->n_l;
     const int n = nl + d.counts->n_h;
     const int stride = gridDim.x * blockDim.x;
@@ -647,7 +647,7 @@ __global__ void k_cell_reorder(DevState d, const uint32_t* __restrict__ skeys,
         nt_store(&d.spos[k], x.x, x.y);
         nt_store(&d.sxv[k], x.x, x.y, v.x, v.y);
         d.sstate[k] = sst;
-        d.sidx[u] = k;
+        if (with_sidx) d.sidx[u] = k;
         const int s = d.cscene[key];
         d.sscene[k] = (uint16_t)s;
         if (!(sst & 1u)) {
@@ -958,7 +958,7 @@ __global__ __launch_bounds__(kSmallSortThreads) void k_sort_small(DevState d, in
 // Cell ranges (R:1300-1340) + reorder into cell-sorted SoA + scene segments.
 // ---------------------------------------------------------------------------
 __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
-                          const uint32_t* __restrict__ vals) {
+                          const uint32_t* __restrict__ vals, int with_sidx) {
     const int nl = d.counts->n_l;
     const int n = nl + d.counts->n_h;
     GRID_STRIDE(k, n) {
@@ -975,7 +975,7 @@ __global__ void k_reorder(DevState d, const uint32_t* __restrict__ keys,
         d.spos[k] = x;
         d.sxv[k] = make_float4(x.x, x.y, v.x, v.y);
         d.sstate[k] = st;
-        d.sidx[u] = k;
+        if (with_sidx) d.sidx[u] = k;
         // the scene owns the cell (cells of a scene are one range)
         const int s = d.cscene[key];
         d.sscene[k] = (uint16_t)s;
@@ -2438,8 +2438,11 @@ void launch_fill_int(int* p, int value, long long n, void* stream) {
 
 // Grid build: clears, hash, sort, ranges + reorder. Returns the buffer index
 // holding the sorted keys/vals.
+// with_sidx: also the sorted index of each id (sidx), which only the band
+// rule's merge test, the relaxation of new children and the neighbour
+// inspection read
 static int enqueue_grid(const DevState& d, const StepShape& sh,
-                        hipStream_t st) {
+                        hipStream_t st, int with_sidx) {
     const int par = sh.list_parity;
     k_begin_hash<<<sh.clear_blocks + sh.grid_elem, 256, 0, st>>>(
         d, sh.clear_blocks, sh.count_sort, sh.solo, d.list_l[par], d.list_h[par], d.keys[0],
@@ -2449,7 +2452,7 @@ static int enqueue_grid(const DevState& d, const StepShape& sh,
         k_cell_ranges<<<sh.cell_tiles, kCellThreads, 0, st>>>(d, sh.key_cells);
         k_cell_scatter<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[0], d.list_l[par],
                                                      d.list_h[par], d.keys[1]);
-        k_cell_reorder<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[1], d.vals[1]);
+        k_cell_reorder<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[1], d.vals[1], with_sidx);
         return 1;
     }
     int cur = 0;
@@ -2468,12 +2471,12 @@ static int enqueue_grid(const DevState& d, const StepShape& sh,
             d.counts, shift, d.hist, d.digit_tot, sh.sort_tiles);
         cur ^= 1;
     }
-    k_reorder<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[cur], d.vals[cur]);
+    k_reorder<<<sh.grid_elem, 256, 0, st>>>(d, d.keys[cur], d.vals[cur], with_sidx);
     return cur;
 }
 
 void launch_grid_only(const DevState& d, const StepShape& sh, void* stream) {
-    enqueue_grid(d, sh, (hipStream_t)stream);
+    enqueue_grid(d, sh, (hipStream_t)stream, 0);
 }
 
 void launch_step(const DevState& d, const StepShape& sh, void* stream,
@@ -2486,7 +2489,7 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     };
     const int par = sh.list_parity;
     mark(0);
-    const int cur = enqueue_grid(d, sh, st);
+    const int cur = enqueue_grid(d, sh, st, sh.band_any || sh.relax_iters > 0);
     const uint32_t* vals = d.vals[cur];
     const uint32_t* keys = d.keys[cur];
     for (int it = 0; it < sh.relax_iters; ++it) {
@@ -2558,7 +2561,7 @@ void launch_filter_scene(const DevState& d, const StepShape& sh, int scene,
 void launch_work_counts(const DevState& d, const StepShape& sh,
                         unsigned long long* out, void* stream) {
     hipStream_t st = (hipStream_t)stream;
-    enqueue_grid(d, sh, st);
+    enqueue_grid(d, sh, st, 0);
     (void)hipMemsetAsync(out, 0, kWorkCounts * sizeof(unsigned long long), st);
     k_work<<<sh.grid_elem, 256, 0, st>>>(d, out);
 }
@@ -2567,7 +2570,7 @@ void launch_work_counts(const DevState& d, const StepShape& sh,
 // density and force kernels in their recording variant.
 void launch_pair_record(const DevState& d, const StepShape& sh, void* stream) {
     hipStream_t st = (hipStream_t)stream;
-    const int cur = enqueue_grid(d, sh, st);
+    const int cur = enqueue_grid(d, sh, st, 0);
     const uint32_t* vals = d.vals[cur];
     const uint32_t* keys = d.keys[cur];
     const int gl = blocks_for(sh.cap_alive, 256, 16384);
@@ -2583,7 +2586,7 @@ void launch_neighbors_n(const DevState& d, const StepShape& sh, int scene,
                         int level, int n, uint32_t* counts, uint32_t* sums,
                         void* stream) {
     hipStream_t st = (hipStream_t)stream;
-    const int cur = enqueue_grid(d, sh, st);
+    const int cur = enqueue_grid(d, sh, st, 1);
     const int par = sh.list_parity;
     const int* list = level ? d.list_h[par] : d.list_l[par];
     k_neighbors<<<blocks_for(n, 256, 4096), 256, 0, st>>>(d, d.vals[cur], list,
