@@ -1315,7 +1315,7 @@ __device__ bool in_band(const DevState& d, const SceneDev& sc, float2 x,
 
 // Split candidates over the sorted low particles (R:1834-1836 rule or the
 // surface band rule).
-__global__ void k_split_cand(DevState d, const uint32_t* __restrict__ vals) {
+__global__ void k_split_cand(DevState d) {
     const int nl = d.counts->n_l;
     GRID_STRIDE(k, nl) {
         const int s = d.sscene[k];
@@ -1323,7 +1323,10 @@ __global__ void k_split_cand(DevState d, const uint32_t* __restrict__ vals) {
         uint8_t cand = 0;
         if (sc.mode != 0 && d.sstate[k] == kActive) {
             if (sc.mode == 1) {
-                cand = d.pos[vals[k]].x > sc.split_x;  // predicted x' (R:1835)
+                // predicted x' (R:1835): the force pass's post-step
+                // position, read in sorted order (the same value it
+                // stored by id)
+                cand = d.snew[k].x > sc.split_x;
             } else {
                 cand = in_band(d, sc, d.spos[k], sc.r_band, sc.t_band, d.scnt[s].hsurf != 0);
             }
@@ -2509,7 +2512,7 @@ void launch_step(const DevState& d, const StepShape& sh, void* stream,
     mark(5);
     k_reduce<<<sh.red_slots, 256, 0, st>>>(d);
     mark(6);
-    k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d, vals);
+    k_split_cand<<<sh.grid_elem, 256, 0, st>>>(d);
     // few groups and long band tests: a lane per child, small blocks to
     // spread them over the CUs
     if (sh.band_any)
