@@ -85,13 +85,13 @@ def count_sort_bytes(n: int, cells: int) -> float:
 
 
 def grid_sort_path(cap_low: int, cells: int) -> str:
-    """The grid build's sort as TSS_SORT_AUTO picks it (include/tss.h):
-    one workgroup up to 65536 keys, the counting sort when the cells number
-    at most 4 per key, else the radix sort."""
+    """The grid build's sort as TSS_SORT_AUTO picks it (include/tss.h): the
+    counting sort when the cells number at most 4 per key, else one
+    workgroup up to 65536 keys, else the radix sort."""
     keys = 5 * max(cap_low, 1)
-    if keys <= 65536:
-        return "small"
-    return "count" if cells <= 4 * keys else "radix"
+    if cells <= 4 * keys:
+        return "count"
+    return "small" if keys <= 65536 else "radix"
 
 
 def _traffic(kernel: str) -> dict | None:

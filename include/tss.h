@@ -259,9 +259,10 @@ typedef struct tss_tuning {
  * TSS_SORT_COUNT the counting sort by cell (counts, prefix over cells,
  * scatter, rank within each cell by list position); TSS_SORT_SMALL the
  * one-workgroup radix sort, for resident capacities of at most 13107 low
- * ids (65536 keys; TSS_E_INVALID otherwise). AUTO takes SMALL where it fits,
- * else COUNT when the resident cells number at most 4 per key of capacity,
- * else RADIX. Every path gives the same order, cell ranges and results. */
+ * ids (65536 keys; TSS_E_INVALID otherwise). AUTO takes COUNT when the
+ * resident cells number at most 4 per key of capacity, else SMALL where it
+ * fits, else RADIX. Every path gives the same order, cell ranges and
+ * results. */
 #define TSS_SORT_AUTO 0
 #define TSS_SORT_RADIX 1
 #define TSS_SORT_COUNT 2

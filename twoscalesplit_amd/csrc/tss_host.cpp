@@ -475,10 +475,6 @@ struct tss_batch {
         sh.scan_tiles_l = (int)((cap + tss::kScanTile - 1) / tss::kScanTile);
         sh.scan_tiles_h = (int)((4 * cap + tss::kScanTile - 1) / tss::kScanTile);
         sh.scan_tiles_g = sh.scan_tiles_l;
-        // a forced one-workgroup sort that no longer fits (scenes added
-        // since) falls back to the automatic choice
-        const bool small_fits = U <= tss::kSmallSortKeys;
-        sh.small_sort = small_fits && (sort_path == TSS_SORT_SMALL || sort_path == TSS_SORT_AUTO);
         sh.cap_binds = 0;
         sh.band_any = 0;
         for (const SceneHost& s : scenes) {
@@ -494,9 +490,21 @@ struct tss_batch {
         sh.passes = (bits + tss::kRadixBits - 1) / tss::kRadixBits;
         sh.key_cells = (int)maxkey;
         sh.cell_tiles = (int)((maxkey + tss::kCellTile - 1) / tss::kCellTile);
-        sh.count_sort = sort_path == TSS_SORT_COUNT ||
-                        (sort_path == TSS_SORT_AUTO && !sh.small_sort &&
-                         maxkey <= tss::kCountCellsPerKey * U);
+        // the grid build's sort: the counting sort where the resident cells
+        // number at most kCountCellsPerKey per key (five short launches,
+        // faster than the one-workgroup radix sort even on small batches:
+        // 0.104 vs 0.128 ms per C1/C2 step), else the one-workgroup radix
+        // sort where the keys fit, else the multi-tile radix sort; a forced
+        // one-workgroup sort that no longer fits (scenes added since) falls
+        // back to this choice
+        const bool small_fits = U <= tss::kSmallSortKeys;
+        const bool count_ok = maxkey <= tss::kCountCellsPerKey * U;
+        int path = sort_path;
+        if (path == TSS_SORT_SMALL && !small_fits) path = TSS_SORT_AUTO;
+        if (path == TSS_SORT_AUTO)
+            path = count_ok ? TSS_SORT_COUNT : small_fits ? TSS_SORT_SMALL : TSS_SORT_RADIX;
+        sh.count_sort = path == TSS_SORT_COUNT;
+        sh.small_sort = path == TSS_SORT_SMALL;
         sh.sorted_buf = sh.count_sort ? 1 : (sh.passes & 1);
         sh.solo = -1;
         int resident = 0;
