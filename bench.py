@@ -9,7 +9,7 @@ reference's split rule with the threshold left of the domain), giving
 adaptive dt, FP32 with compensated sums, hipGraph-captured steps.
 
 A step advances every low and high particle once: grid build (hash, stable
-radix sort, cell ranges, reorder), density + Tait, forces + symplectic
+sort by cell, cell ranges, reorder), density + Tait, forces + symplectic
 Euler, adaptivity + list compaction, FP64 reductions (dt, conservation).
 value = particle-steps/s = steps * (n_low + n_high) / wall time, with the
 state resident in HBM.
